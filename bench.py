@@ -1,0 +1,217 @@
+"""Throughput benchmark: 2-field lphi4 preheating on a 256^3 lattice, leapfrog, FP32-bulk vs FP64.
+
+Workload (one MI355X): inflaton phi (V = phi^4/4) plus daughter chi coupled by q/2 phi^2 chi^2 with
+q = 100, program units lambda = 1, phi0 = 1, pi0 = -0.5, fStar/MPl = 2.34 (so the self-consistent
+expansion is live), periodic 256^3, kIR = 0.05, dt = 0.005, seeded vacuum Gaussian fluctuations
+on both fields (cut at k = N kIR / 4).  warmup + steps = 4000 leapfrog steps by default (t = 20).
+
+A step is one leapfrog step of the whole lattice including the volume reductions and the
+self-consistent scale-factor update (one fused lattice pass + one background-update launch).
+site-updates/s = N^3 * steps / seconds of the timed loop (inputs resident in HBM).
+
+Printed: ONE JSON line (rank 0).  value = FP32-bulk site-updates/s; the FP64 drop-in mode, the
+accuracy of FP32-bulk against FP64 (Friedmann-constraint violation, band-averaged spectrum error),
+the roofline of the dominant kernel (HIP-event timed, algorithmic bytes) and the CPU baseline
+(the FP64 oracle, reference-structured, on this host's cores) are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "site-updates/s + %HBM peak, lphi4 256^3: FP32-bulk vs FP64 at stated energy error"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_lattice(clmi, spec, N, kIR, dt, precision, seed):
+    dx = 2 * math.pi / kIR / N
+    lat = clmi.Lattice(spec, N, dx, dt, precision, expansion=True)
+    # ScaleFactorInitializer (initializers/scalefactorinitializer.h:49-64): aDot from the
+    # homogeneous energy, then the fluctuations are added with that aDot
+    phi0 = spec.fld0
+    pi0 = spec.pi0
+    kin = sum(0.5 * p * p for p in pi0)
+    rho_hom = kin + spec.potential(phi0)
+    aDot0 = spec.fStar / spec.MPl * math.sqrt(rho_hom / 3.0)
+    lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=aDot0)
+    m2 = spec.mass2(phi0)
+    kcut = N * kIR / 4
+    for f in range(spec.n_scalars):
+        lat.grf_scalar(f, seed, f, m2[f], aDot0, kIR, kcut, phi0[f], pi0[f])
+    # ModelInitializer (initializers/modelinitializer.h:71-80): aDotI = sqrt(Friedmann RHS)
+    _, _, rhs = lat.hubble_constraint()
+    lat.set_background(aDotI=math.sqrt(rhs))
+    lat.lf_init()
+    return lat, dx
+
+
+def timed_run(lat, warmup, steps):
+    lat.lf_evolve(warmup)
+    lat.synchronize()
+    lat.profile_enable(True)
+    t0 = time.perf_counter()
+    lat.lf_evolve(steps)
+    lat.synchronize()
+    t1 = time.perf_counter()
+    ms, launches, bytes_per = lat.profile_read()
+    lat.profile_enable(False)
+    return t1 - t0, ms, launches, bytes_per
+
+
+def band_spectrum_error(lat_a, lat_b, kind, index, N, kIR, dk=1.0, nband=8):
+    from cosmolattice_amd.clmi import power_spectrum
+
+    sa, ca, ma = lat_a.spectrum_bins(kind, index, 0, dk)
+    sb, cb, mb = lat_b.spectrum_bins(kind, index, 0, dk)
+    _, pa = power_spectrum(sa, ca, ma, N, kIR, dk, 1, 1)
+    _, pb = power_spectrum(sb, cb, mb, N, kIR, dk, 1, 1)
+    pa, pb = pa[1:], pb[1:]
+    edges = np.linspace(0, len(pb), nband + 1).astype(int)
+    errs = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        ref = pb[lo:hi].sum()
+        if ref > 0:
+            errs.append(abs(pa[lo:hi].sum() - ref) / ref)
+    return float(np.mean(errs)), float(np.max(errs))
+
+
+def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
+    """FP64 oracle (reference-structured: one lattice sweep per reference expression) timed on this
+    host's cores on a bounded sample of the same workload, starting from the GPU run's state."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    from cosmolattice_amd.clmi import FLD_S, PI_S
+
+    ref = orc.OracleLattice(spec, N, dx, dt, expansion=True)
+    lat64.lf_sync()
+    for f in range(spec.n_scalars):
+        ref.upload(FLD_S, f, 0, lat64.download(FLD_S, f, 0))
+        ref.upload(PI_S, f, 0, lat64.download(PI_S, f, 0))
+    ref.set_background(**lat64.background())
+    ref.lf_init()
+    ref.lf_evolve(1)  # first (half-kick) step outside the timing
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ref.lf_evolve(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 50:
+            break
+    return {
+        "value": N ** 3 * steps / el,
+        "unit": "site-updates/s",
+        "cores": orc.num_threads(),
+        "kind": "port",
+        "sample": f"{steps} leapfrog steps of the same 256^3 2-field lphi4 lattice (FP64, reference-structured "
+                  f"C/OpenMP oracle: 7 lattice sweeps per step), {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the lattice on a single MI355X "
+              "(multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
+        sys.exit(2)
+
+    from cosmolattice_amd import clmi
+
+    N, kIR, dt, q = args.N, 0.05, 0.005, 100.0
+    spec = clmi.lphi4(q, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+
+    res = {}
+    lats = {}
+    for name, prec in (("f64", clmi.CLMI_F64), ("f32_bulk", clmi.CLMI_F32_BULK)):
+        log(f"[bench] {name}: init {N}^3")
+        lat, dx = make_lattice(clmi, spec, N, kIR, dt, prec, args.seed)
+        log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")
+        sec, kms, launches, bpl = timed_run(lat, args.warmup, args.steps)
+        res[name] = {"seconds": sec, "kernel_ms": kms, "launches": launches, "bytes_per_launch": bpl,
+                     "site_updates_per_s": N ** 3 * args.steps / sec, "ms_per_step": 1e3 * sec / args.steps}
+        log(f"[bench] {name}: {res[name]['ms_per_step']:.4f} ms/step, "
+            f"{res[name]['site_updates_per_s'] / 1e9:.2f} G site-updates/s")
+        lats[name] = lat
+
+    # accuracy of FP32-bulk vs FP64 at the same step
+    from cosmolattice_amd.clmi import FLD_S
+
+    acc = {}
+    for name, lat in lats.items():
+        lat.lf_sync()
+        acc[name + "_friedmann"] = abs(lat.hubble_constraint()[0])
+    se_chi = band_spectrum_error(lats["f32_bulk"], lats["f64"], FLD_S, 1, N, kIR)
+    se_phi = band_spectrum_error(lats["f32_bulk"], lats["f64"], FLD_S, 0, N, kIR)
+    acc["spectrum_chi_band_rel_err_mean"], acc["spectrum_chi_band_rel_err_max"] = se_chi
+    acc["spectrum_phi_band_rel_err_mean"], acc["spectrum_phi_band_rel_err_max"] = se_phi
+    bg32, bg64 = lats["f32_bulk"].background(), lats["f64"].background()
+    acc["a_rel_diff"] = abs(bg32["aI"] - bg64["aI"]) / bg64["aI"]
+    acc["t_end"] = (args.warmup + args.steps) * dt
+
+    r32, r64 = res["f32_bulk"], res["f64"]
+    def roof(r):
+        avg_s = r["kernel_ms"] / 1e3 / max(r["launches"], 1)
+        achieved = r["bytes_per_launch"] / avg_s / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_scalar_pass (fused drift+kick+reductions)",
+                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"]}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle)")
+        cpu = cpu_baseline(lats["f64"], spec, N, 2 * math.pi / kIR / N, dt, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": r32["site_updates_per_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded vacuum Gaussian random fields (Philox4x32-10) on both fields",
+        "config": {"workload": "lphi4 2-field preheating 256^3 leapfrog (q=100, kIR=0.05, dt=0.005, "
+                               "self-consistent expansion)", "N": N, "kIR": kIR, "dt": dt, "q": q,
+                   "mode": "FP32-bulk: fields f32 hi/lo, momenta f32, reductions + scale factor f64",
+                   "fStar_over_MPl": spec.fStar / spec.MPl},
+        "roofline": roof(r32),
+        "cpu_baseline": cpu,
+        "fp64": {"value": r64["site_updates_per_s"], "unit": "site-updates/s", "ms_per_step": r64["ms_per_step"],
+                 "roofline": roof(r64)},
+        "f32_over_f64": r32["site_updates_per_s"] / r64["site_updates_per_s"],
+        "hbm_frac_f32": roof(r32)["frac"],
+        "accuracy": acc,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

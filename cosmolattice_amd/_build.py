@@ -1,0 +1,84 @@
+"""Build recipe for libclmi.so (HIP, gfx950).
+
+The library is compiled in-tree so that the built .so travels with the source tree.  Every HIP
+translation unit is compiled with FP contraction off (the FP64 kernels then evaluate the
+reference's expressions in the reference's operation order, without fused multiply-adds) and
+with packed-FP32 math disabled.
+"""
+from __future__ import annotations
+
+import concurrent.futures as _cf
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+LIB = PKG / "libclmi.so"
+OBJDIR = PKG / "_objs"
+
+HIP_SOURCES = [
+    "scalar_pass.hip",
+    "gauge_pass.hip",
+    "background.hip",
+    "fft_kernels.hip",
+    "clmi_api.hip",
+]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+]
+
+
+def _hipcc() -> str:
+    h = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(h).exists():
+        raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    return h
+
+
+def _compile(src: Path, obj: Path) -> None:
+    cmd = [_hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+
+
+def build_library(verbose: bool = True) -> Path:
+    """Compile every HIP source for gfx950 and link libclmi.so (always rebuilds)."""
+    OBJDIR.mkdir(exist_ok=True)
+    objs = []
+    jobs = min(len(HIP_SOURCES), max(1, (os.cpu_count() or 2)))
+    with _cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = []
+        for name in HIP_SOURCES:
+            src = CSRC / name
+            obj = OBJDIR / (name + ".o")
+            objs.append(obj)
+            futs.append(ex.submit(_compile, src, obj))
+        for f in futs:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build_library()

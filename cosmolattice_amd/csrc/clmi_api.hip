@@ -1,0 +1,881 @@
+// C ABI of libclmi.so (declared in include/clmi.h): context management, pass orchestration,
+// data movement, spectra and initial conditions.
+//
+// Leapfrog schedule.  The reference's step (evolvers/leapfrog.h:42-89) is
+//   kick(aI_n) -> <pi^2> -> scale-factor kick+drift -> drift(aSI) -> <grad^2>, <V>.
+// Here the drift of step n is deferred and executed inside the next lattice pass, which then
+// computes the drifted fields (at every stencil point), their reductions, the kick of step n+1 on
+// them and <pi^2> of the kicked momenta, all in one read and one write of every array.  The
+// reductions of that pass are exactly the storeFieldsAverages of step n and the
+// storeMomentaAverages of step n+1, so the background update that follows reproduces the
+// reference's sequence of scalar operations.  Any call that observes the fields first flushes the
+// deferred drift with a drift-only pass.
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "clmi_internal.hpp"
+
+using namespace clmi;
+
+namespace {
+thread_local std::string g_create_error;
+
+enum Sector { SEC_S = 0, SEC_CS = 1 };
+}  // namespace
+
+struct clmi_ctx {
+  clmi_config cfg{};
+  int N = 0;
+  uint32_t V = 0;
+  bool f32 = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // double-buffered arrays [array][buffer]
+  DevArr fS[CLMI_MAX_SCALARS][2], pS[CLMI_MAX_SCALARS][2];
+  DevArr fCS[2][2], pCS[2][2], fU[3][2], pU[3][2];
+  int cur_fS = 0, cur_pS = 0, cur_fG = 0, cur_pG = 0;  // fG: phiCS and A together; pG: piCS and piA
+
+  DevState *dstate = nullptr;
+  double *partials = nullptr;
+  double *dscratch = nullptr;  // small device scratch for row sums
+  int blocks_s = 0, blocks_g = 0;
+
+  ScalarPoly spoly{};
+  CsPoly cpoly{};
+  ModelConsts mc{};
+
+  bool synced = true;
+  bool pending_drift = false;
+
+  // FFT resources (lazy)
+  double *stage = nullptr;  // N^3 doubles
+  void *fk = nullptr;       // N*N*(N/2+1) double2
+  void *fk2 = nullptr;
+  hipfftHandle plan_r2c = 0, plan_c2r = 0;
+  bool have_r2c = false, have_c2r = false;
+  BinTable bins;
+  double *bin_out = nullptr;
+  int bin_out_n = 0;
+
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;  // pairs
+  size_t ev_used = 0;
+  double prof_ms = 0.0;
+  int64_t prof_launches = 0;
+
+  std::string err;
+};
+
+namespace {
+
+int fail(clmi_ctx *c, int code, const std::string &msg)
+{
+  if (c)
+    c->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                                     \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess) return fail((c), CLMI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define FFTCHK(c, expr)                                                                              \
+  do {                                                                                               \
+    hipfftResult r_ = (expr);                                                                        \
+    if (r_ != HIPFFT_SUCCESS) return fail((c), CLMI_E_HIP, std::string(#expr) + " failed: " + std::to_string((int)r_)); \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != CLMI_OK) return rc_; \
+  } while (0)
+
+size_t bytes_field(bool f32) { return f32 ? 4 : 8; }
+
+int alloc_arr(clmi_ctx *c, DevArr &a, bool field)
+{
+  const size_t n = c->V;
+  const size_t b = bytes_field(c->f32);
+  HIPCHK(c, hipMalloc(&a.p, n * b));
+  HIPCHK(c, hipMemsetAsync(a.p, 0, n * b, c->stream));
+  if (c->f32 && field) {
+    HIPCHK(c, hipMalloc((void **)&a.lo, n * sizeof(float)));
+    HIPCHK(c, hipMemsetAsync(a.lo, 0, n * sizeof(float), c->stream));
+  }
+  return CLMI_OK;
+}
+
+void free_arr(DevArr &a)
+{
+  if (a.p) (void)hipFree(a.p);
+  if (a.lo) (void)hipFree(a.lo);
+  a.p = nullptr;
+  a.lo = nullptr;
+}
+
+// Algorithmic HBM bytes of one fused step pass: every evolved array read once and written once.
+double fused_pass_bytes(const clmi_ctx *c)
+{
+  const double bf = c->f32 ? 8.0 : 8.0;  // field: FP64, or FP32 hi + FP32 lo
+  const double bp = c->f32 ? 4.0 : 8.0;  // momentum
+  double per_site = c->cfg.n_scalars * 2.0 * (bf + bp);
+  if (c->cfg.n_cscalars > 0) per_site += 2 * 2.0 * (bf + bp);
+  if (c->cfg.n_u1 > 0) per_site += 3 * 2.0 * (bf + bp);
+  return per_site * (double)c->V;
+}
+
+int prof_flush(clmi_ctx *c)
+{
+  if (c->ev_used == 0) return CLMI_OK;
+  HIPCHK(c, hipEventSynchronize(c->ev[c->ev_used - 1]));
+  for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    c->prof_ms += ms;
+    c->prof_launches += 1;
+  }
+  c->ev_used = 0;
+  return CLMI_OK;
+}
+
+int prof_event(clmi_ctx *c)
+{
+  if (c->ev_used == c->ev.size()) {
+    if (c->ev.size() >= 2048) {
+      CHK(prof_flush(c));
+    } else {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->ev.push_back(e);
+    }
+  }
+  HIPCHK(c, hipEventRecord(c->ev[c->ev_used], c->stream));
+  c->ev_used++;
+  return CLMI_OK;
+}
+
+// Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
+int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
+{
+  const bool timed = c->prof && drift && kick;
+  if (timed) CHK(prof_event(c));
+  int r = 0;
+  if (c->cfg.n_scalars > 0) {
+    ScalarPassArgs a{};
+    const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
+    const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
+    for (int f = 0; f < c->cfg.n_scalars; ++f) {
+      a.phi_in[f] = c->fS[f][fi].p;
+      a.philo_in[f] = c->fS[f][fi].lo;
+      a.pi_in[f] = c->pS[f][pi].p;
+      a.phi_out[f] = c->fS[f][fo].p;
+      a.philo_out[f] = c->fS[f][fo].lo;
+      a.pi_out[f] = c->pS[f][po].p;
+    }
+    a.N = c->N;
+    a.V = c->V;
+    a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
+    a.inv_dx = 1.0 / c->cfg.dx;
+    a.cdt = cdt_s;
+    a.ckick = ckick_s;
+    a.st = c->dstate;
+    a.partials = c->partials + (size_t)r * kNumRed;
+    a.poly = c->spoly;
+    HIPCHK(c, launch_scalar_pass(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->stream));
+    if (red) r += c->blocks_s;
+    c->cur_fS = fo;
+    c->cur_pS = po;
+  }
+  if (c->cfg.n_cscalars > 0) {
+    GaugePassArgs a{};
+    const int fi = c->cur_fG, fo = drift ? 1 - c->cur_fG : c->cur_fG;
+    const int pi = c->cur_pG, po = (drift && kick) ? 1 - c->cur_pG : c->cur_pG;
+    for (int k = 0; k < 2; ++k) {
+      a.phi_in[k] = c->fCS[k][fi].p;
+      a.philo_in[k] = c->fCS[k][fi].lo;
+      a.pi_in[k] = c->pCS[k][pi].p;
+      a.phi_out[k] = c->fCS[k][fo].p;
+      a.philo_out[k] = c->fCS[k][fo].lo;
+      a.pi_out[k] = c->pCS[k][po].p;
+    }
+    for (int d = 0; d < 3; ++d)
+      if (c->cfg.n_u1 > 0) {
+        a.A_in[d] = c->fU[d][fi].p;
+        a.Alo_in[d] = c->fU[d][fi].lo;
+        a.piA_in[d] = c->pU[d][pi].p;
+        a.A_out[d] = c->fU[d][fo].p;
+        a.Alo_out[d] = c->fU[d][fo].lo;
+        a.piA_out[d] = c->pU[d][po].p;
+      }
+    a.N = c->N;
+    a.V = c->V;
+    a.has_u1 = c->cfg.n_u1 > 0;
+    a.dx = c->cfg.dx;
+    a.inv_dx = 1.0 / c->cfg.dx;
+    a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
+    a.gQ = c->cfg.gQ;
+    const double fw = c->cfg.fStar / c->cfg.omegaStar;
+    a.jnorm = 2.0 / c->cfg.dx * (fw * fw);
+    a.cdt = cdt_s;
+    a.ckick = ckick_g;
+    a.st = c->dstate;
+    a.partials = c->partials + (size_t)r * kNumRed;
+    a.poly = c->cpoly;
+    HIPCHK(c, launch_gauge_pass(c->f32, drift, kick, red, a, c->blocks_g, c->stream));
+    if (red) r += c->blocks_g;
+    c->cur_fG = fo;
+    c->cur_pG = po;
+  }
+  if (timed) CHK(prof_event(c));
+  *rows = r;
+  return CLMI_OK;
+}
+
+int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> ops)
+{
+  UpdateOps u{};
+  for (const auto &o : ops) {
+    u.op[u.n] = o.first;
+    u.w[u.n] = o.second;
+    u.n++;
+  }
+  HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
+  return CLMI_OK;
+}
+
+// Apply the deferred drift of the last leapfrog step (drift-only pass + storeFieldsAverages).
+int flush(clmi_ctx *c)
+{
+  if (!c->pending_drift) return CLMI_OK;
+  const bool ex = c->cfg.expansion != 0;
+  int rows = 0;
+  CHK(run_pass(c, true, false, ex, c->cfg.dt, 0.0, 0.0, &rows));
+  if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}}));
+  c->pending_drift = false;
+  return CLMI_OK;
+}
+
+int check_ctx(clmi_ctx *c)
+{
+  if (!c) return fail(nullptr, CLMI_E_INVALID, "null context");
+  HIPCHK(c, hipSetDevice(c->device));
+  return CLMI_OK;
+}
+
+int select_arr(clmi_ctx *c, int kind, int index, int comp, DevArr **out, bool *is_field)
+{
+  const clmi_config &g = c->cfg;
+  switch (kind) {
+    case CLMI_FLD_S:
+    case CLMI_PI_S:
+      if (index < 0 || index >= g.n_scalars || comp != 0) break;
+      *is_field = kind == CLMI_FLD_S;
+      *out = kind == CLMI_FLD_S ? &c->fS[index][c->cur_fS] : &c->pS[index][c->cur_pS];
+      return CLMI_OK;
+    case CLMI_FLD_CS:
+    case CLMI_PI_CS:
+      if (index != 0 || g.n_cscalars < 1 || comp < 0 || comp > 1) break;
+      *is_field = kind == CLMI_FLD_CS;
+      *out = kind == CLMI_FLD_CS ? &c->fCS[comp][c->cur_fG] : &c->pCS[comp][c->cur_pG];
+      return CLMI_OK;
+    case CLMI_FLD_U1:
+    case CLMI_PI_U1:
+      if (index != 0 || g.n_u1 < 1 || comp < 0 || comp > 2) break;
+      *is_field = kind == CLMI_FLD_U1;
+      *out = kind == CLMI_FLD_U1 ? &c->fU[comp][c->cur_fG] : &c->pU[comp][c->cur_pG];
+      return CLMI_OK;
+    default:
+      break;
+  }
+  return fail(c, CLMI_E_INVALID,
+              "no such field: kind " + std::to_string(kind) + " index " + std::to_string(index) + " comp " +
+                  std::to_string(comp));
+}
+
+int ensure_stage(clmi_ctx *c)
+{
+  if (!c->stage) HIPCHK(c, hipMalloc(&c->stage, sizeof(double) * (size_t)c->V));
+  return CLMI_OK;
+}
+
+int ensure_fft(clmi_ctx *c, bool r2c, bool c2r, bool two)
+{
+  CHK(ensure_stage(c));
+  const size_t nm = (size_t)c->N * c->N * (c->N / 2 + 1);
+  if (!c->fk) HIPCHK(c, hipMalloc(&c->fk, 16 * nm));
+  if (two && !c->fk2) HIPCHK(c, hipMalloc(&c->fk2, 16 * nm));
+  if (r2c && !c->have_r2c) {
+    FFTCHK(c, hipfftPlan3d(&c->plan_r2c, c->N, c->N, c->N, HIPFFT_D2Z));
+    FFTCHK(c, hipfftSetStream(c->plan_r2c, c->stream));
+    c->have_r2c = true;
+  }
+  if (c2r && !c->have_c2r) {
+    FFTCHK(c, hipfftPlan3d(&c->plan_c2r, c->N, c->N, c->N, HIPFFT_Z2D));
+    FFTCHK(c, hipfftSetStream(c->plan_c2r, c->stream));
+    c->have_c2r = true;
+  }
+  return CLMI_OK;
+}
+
+int read_state(clmi_ctx *c, DevState *h)
+{
+  HIPCHK(c, hipMemcpyAsync(h, c->dstate, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clmi_create(const clmi_config *cfg, clmi_ctx **out)
+{
+  if (!cfg || !out) return fail(nullptr, CLMI_E_INVALID, "null argument");
+  *out = nullptr;
+  const clmi_config &g = *cfg;
+  if (g.N < 4 || g.N > 1024 || (g.N & 1)) return fail(nullptr, CLMI_E_INVALID, "N must be even, 4 <= N <= 1024");
+  if (g.n_scalars < 0 || g.n_scalars > CLMI_MAX_SCALARS) return fail(nullptr, CLMI_E_INVALID, "n_scalars out of range");
+  if (g.n_cscalars < 0 || g.n_cscalars > 1) return fail(nullptr, CLMI_E_INVALID, "n_cscalars must be 0 or 1");
+  if (g.n_u1 < 0 || g.n_u1 > 1 || (g.n_u1 == 1 && g.n_cscalars != 1))
+    return fail(nullptr, CLMI_E_INVALID, "n_u1 must be 0 or 1 and requires one complex scalar");
+  if (g.n_scalars + g.n_cscalars == 0) return fail(nullptr, CLMI_E_INVALID, "model has no fields");
+  if (g.precision != CLMI_F64 && g.precision != CLMI_F32_BULK) return fail(nullptr, CLMI_E_INVALID, "bad precision");
+  if (!(g.dx > 0) || g.dt == 0.0) return fail(nullptr, CLMI_E_INVALID, "dx must be > 0 and dt != 0");
+  if (g.n_terms < 0 || g.n_terms > CLMI_MAX_TERMS || g.n_cs_terms < 0 || g.n_cs_terms > CLMI_MAX_TERMS)
+    return fail(nullptr, CLMI_E_INVALID, "too many potential terms");
+  for (int t = 0; t < g.n_terms; ++t)
+    for (int f = 0; f < CLMI_MAX_SCALARS; ++f)
+      if (g.term_exp[t][f] < 0 || g.term_exp[t][f] > 16 || (f >= g.n_scalars && g.term_exp[t][f] != 0))
+        return fail(nullptr, CLMI_E_INVALID, "bad exponent in scalar potential term " + std::to_string(t));
+  for (int k = 0; k < g.n_cs_terms; ++k)
+    if (g.cs_pow[k] < 0 || g.cs_pow[k] > 32 || (g.cs_pow[k] & 1))
+      return fail(nullptr, CLMI_E_INVALID, "complex-scalar potential powers must be even, 0..32");
+  if (g.expansion && !(g.MPl > 0 && g.fStar > 0)) return fail(nullptr, CLMI_E_INVALID, "expansion needs MPl, fStar > 0");
+
+  clmi_ctx *c = new clmi_ctx();
+  c->cfg = g;
+  c->N = g.N;
+  c->V = (uint32_t)g.N * g.N * g.N;
+  c->f32 = g.precision == CLMI_F32_BULK;
+  c->device = g.device;
+  auto bail = [&](int rc) {
+    g_create_error = c->err;
+    clmi_destroy(c);
+    return rc;
+  };
+  if (hipSetDevice(g.device) != hipSuccess) {
+    fail(c, CLMI_E_HIP, "hipSetDevice failed");
+    return bail(CLMI_E_HIP);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail(c, CLMI_E_HIP, "hipStreamCreate failed");
+    return bail(CLMI_E_HIP);
+  }
+  int rc = CLMI_OK;
+  for (int f = 0; f < g.n_scalars && rc == CLMI_OK; ++f)
+    for (int b = 0; b < 2 && rc == CLMI_OK; ++b) {
+      rc = alloc_arr(c, c->fS[f][b], true);
+      if (rc == CLMI_OK) rc = alloc_arr(c, c->pS[f][b], false);
+    }
+  for (int k = 0; k < 2 * g.n_cscalars && rc == CLMI_OK; ++k)
+    for (int b = 0; b < 2 && rc == CLMI_OK; ++b) {
+      rc = alloc_arr(c, c->fCS[k][b], true);
+      if (rc == CLMI_OK) rc = alloc_arr(c, c->pCS[k][b], false);
+    }
+  for (int d = 0; d < 3 * g.n_u1 && rc == CLMI_OK; ++d)
+    for (int b = 0; b < 2 && rc == CLMI_OK; ++b) {
+      rc = alloc_arr(c, c->fU[d][b], true);
+      if (rc == CLMI_OK) rc = alloc_arr(c, c->pU[d][b], false);
+    }
+  if (rc != CLMI_OK) return bail(CLMI_E_NOMEM);
+
+  c->blocks_s = scalar_pass_blocks(c->V);
+  c->blocks_g = gauge_pass_blocks(c->V);
+  const size_t rows = (size_t)(c->blocks_s + c->blocks_g) + 16;
+  if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->dstate, sizeof(DevState)) != hipSuccess) {
+    fail(c, CLMI_E_NOMEM, "device allocation failed");
+    return bail(CLMI_E_NOMEM);
+  }
+
+  c->spoly.nterms = g.n_terms;
+  for (int t = 0; t < g.n_terms; ++t) {
+    c->spoly.c[t] = g.term_coeff[t];
+    for (int f = 0; f < CLMI_MAX_SCALARS; ++f) c->spoly.e[t][f] = g.term_exp[t][f];
+  }
+  c->cpoly.nterms = g.n_cs_terms;
+  for (int k = 0; k < g.n_cs_terms; ++k) {
+    c->cpoly.c[k] = g.cs_coeff[k];
+    c->cpoly.h[k] = g.cs_pow[k] / 2;
+  }
+  c->mc.ns = g.n_scalars;
+  c->mc.ncs = g.n_cscalars;
+  c->mc.nu1 = g.n_u1;
+  c->mc.expansion = g.expansion;
+  c->mc.dt = g.dt;
+  c->mc.alpha = g.alpha;
+  c->mc.fStar = g.fStar;
+  c->mc.omegaStar = g.omegaStar;
+  c->mc.MPl = g.MPl;
+  c->mc.invV = 1.0 / ((double)g.N * g.N * g.N);
+
+  DevState h{};
+  h.bg.aI = h.bg.aSI = h.bg.aIM = 1.0;
+  if (hipMemcpyAsync(c->dstate, &h, sizeof(DevState), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    fail(c, CLMI_E_HIP, "state upload failed");
+    return bail(CLMI_E_HIP);
+  }
+  if (update(c, 0, {}) != CLMI_OK || hipStreamSynchronize(c->stream) != hipSuccess) return bail(CLMI_E_HIP);
+  *out = c;
+  return CLMI_OK;
+}
+
+void clmi_destroy(clmi_ctx *c)
+{
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int f = 0; f < CLMI_MAX_SCALARS; ++f)
+    for (int b = 0; b < 2; ++b) {
+      free_arr(c->fS[f][b]);
+      free_arr(c->pS[f][b]);
+    }
+  for (int k = 0; k < 2; ++k)
+    for (int b = 0; b < 2; ++b) {
+      free_arr(c->fCS[k][b]);
+      free_arr(c->pCS[k][b]);
+    }
+  for (int d = 0; d < 3; ++d)
+    for (int b = 0; b < 2; ++b) {
+      free_arr(c->fU[d][b]);
+      free_arr(c->pU[d][b]);
+    }
+  if (c->partials) (void)hipFree(c->partials);
+  if (c->dscratch) (void)hipFree(c->dscratch);
+  if (c->dstate) (void)hipFree(c->dstate);
+  if (c->stage) (void)hipFree(c->stage);
+  if (c->fk) (void)hipFree(c->fk);
+  if (c->fk2) (void)hipFree(c->fk2);
+  if (c->bin_out) (void)hipFree(c->bin_out);
+  free_bin_table(c->bins);
+  if (c->have_r2c) (void)hipfftDestroy(c->plan_r2c);
+  if (c->have_c2r) (void)hipfftDestroy(c->plan_c2r);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *clmi_last_error(const clmi_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int clmi_synchronize(clmi_ctx *c)
+{
+  CHK(check_ctx(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_upload_field(clmi_ctx *c, int kind, int index, int comp, const double *host)
+{
+  CHK(check_ctx(c));
+  if (!host) return fail(c, CLMI_E_INVALID, "null host buffer");
+  CHK(flush(c));
+  DevArr *a = nullptr;
+  bool is_field = false;
+  CHK(select_arr(c, kind, index, comp, &a, &is_field));
+  const size_t n = c->V;
+  if (!c->f32) {
+    HIPCHK(c, hipMemcpyAsync(a->p, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    CHK(ensure_stage(c));
+    HIPCHK(c, hipMemcpyAsync(c->stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (is_field)
+      HIPCHK(c, launch_split_f64(c->stage, (float *)a->p, a->lo, c->V, c->stream));
+    else
+      HIPCHK(c, launch_to_f32(c->stage, (float *)a->p, c->V, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_download_field(clmi_ctx *c, int kind, int index, int comp, double *host)
+{
+  CHK(check_ctx(c));
+  if (!host) return fail(c, CLMI_E_INVALID, "null host buffer");
+  CHK(flush(c));
+  DevArr *a = nullptr;
+  bool is_field = false;
+  CHK(select_arr(c, kind, index, comp, &a, &is_field));
+  const size_t n = c->V;
+  if (!c->f32) {
+    HIPCHK(c, hipMemcpyAsync(host, a->p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    CHK(ensure_stage(c));
+    HIPCHK(c, launch_join_f64((const float *)a->p, a->lo, c->stage, c->V, c->stream));
+    HIPCHK(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_set_background(clmi_ctx *c, const clmi_background *bg)
+{
+  CHK(check_ctx(c));
+  if (!bg) return fail(c, CLMI_E_INVALID, "null background");
+  CHK(flush(c));
+  DevState h{};
+  CHK(read_state(c, &h));
+  h.bg = *bg;
+  HIPCHK(c, hipMemcpyAsync(c->dstate, &h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+  CHK(update(c, 0, {}));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_get_background(clmi_ctx *c, clmi_background *bg)
+{
+  CHK(check_ctx(c));
+  if (!bg) return fail(c, CLMI_E_INVALID, "null background");
+  CHK(flush(c));
+  DevState h{};
+  CHK(read_state(c, &h));
+  *bg = h.bg;
+  return CLMI_OK;
+}
+
+int clmi_lf_init(clmi_ctx *c, int restart)
+{
+  CHK(check_ctx(c));
+  CHK(flush(c));
+  c->synced = restart == 0;
+  if (c->synced && c->cfg.expansion) CHK(update(c, 0, {{OP_LF_INIT, 0.0}}));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_lf_evolve(clmi_ctx *c, int nsteps)
+{
+  CHK(check_ctx(c));
+  if (nsteps < 0) return fail(c, CLMI_E_INVALID, "nsteps < 0");
+  const bool ex = c->cfg.expansion != 0;
+  const double dt = c->cfg.dt;
+  for (int n = 0; n < nsteps; ++n) {
+    const double w = c->synced ? 0.5 : 1.0;
+    int rows = 0;
+    if (c->pending_drift) {
+      CHK(run_pass(c, true, true, ex, dt, w * dt, w * dt, &rows));
+      if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}));
+    } else {
+      CHK(run_pass(c, false, true, ex, 0.0, w * dt, w * dt, &rows));
+      if (ex) CHK(update(c, rows, {{OP_LF_MOMENTA, 0.0}}));
+    }
+    c->pending_drift = true;
+    c->synced = false;
+  }
+  return CLMI_OK;
+}
+
+int clmi_lf_sync(clmi_ctx *c)
+{
+  CHK(check_ctx(c));
+  CHK(flush(c));
+  if (!c->synced) {
+    const bool ex = c->cfg.expansion != 0;
+    int rows = 0;
+    CHK(run_pass(c, false, true, ex, 0.0, 0.5 * c->cfg.dt, 0.5 * c->cfg.dt, &rows));
+    if (ex) CHK(update(c, rows, {{OP_LF_SYNC, 0.0}}));
+  }
+  c->synced = true;
+  return CLMI_OK;
+}
+
+int clmi_lf_is_synced(const clmi_ctx *c) { return c && c->synced ? 1 : 0; }
+
+int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
+{
+  CHK(check_ctx(c));
+  if (!ws || nws < 1 || nsteps < 0) return fail(c, CLMI_E_INVALID, "bad composition weights");
+  CHK(flush(c));
+  const bool ex = c->cfg.expansion != 0;
+  const double dt = c->cfg.dt;
+  const int stages = (nws - 1) * 2 + 1;
+  for (int n = 0; n < nsteps; ++n)
+    for (int s = 0; s < stages; ++s) {
+      const double w = ws[s < nws ? s : stages - s - 1];
+      int rows = 0;
+      if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
+      CHK(run_pass(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, &rows));
+      if (ex) CHK(update(c, rows, {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
+      CHK(run_pass(c, true, false, ex, dt * w, 0.0, 0.0, &rows));
+      if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}}));
+      CHK(run_pass(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, &rows));
+      if (ex) CHK(update(c, rows, {{OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
+    }
+  return CLMI_OK;
+}
+
+int clmi_kick(clmi_ctx *c, double ck)
+{
+  CHK(check_ctx(c));
+  CHK(flush(c));
+  int rows = 0;
+  CHK(run_pass(c, false, true, false, 0.0, ck, ck, &rows));
+  return CLMI_OK;
+}
+
+int clmi_drift(clmi_ctx *c, double cdt)
+{
+  CHK(check_ctx(c));
+  CHK(flush(c));
+  int rows = 0;
+  CHK(run_pass(c, true, false, false, cdt, 0.0, 0.0, &rows));
+  return CLMI_OK;
+}
+
+int clmi_set_all_averages(clmi_ctx *c)
+{
+  CHK(check_ctx(c));
+  CHK(flush(c));
+  int rows = 0;
+  CHK(run_pass(c, false, false, true, 0.0, 0.0, 0.0, &rows));
+  CHK(update(c, rows, {{OP_SET_ALL, 0.0}}));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_volume_averages(clmi_ctx *c, clmi_averages *out)
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  CHK(flush(c));
+  int rows = 0;
+  CHK(run_pass(c, false, false, true, 0.0, 0.0, 0.0, &rows));
+  CHK(update(c, rows, {}));
+  DevState h{};
+  CHK(read_state(c, &h));
+  out->pi2S = h.red[RED_PI2S];
+  out->grad2S = h.red[RED_GRAD2S];
+  out->pot = h.red[RED_POT];
+  out->pi2CS = h.red[RED_PI2CS];
+  out->grad2CS = h.red[RED_GRAD2CS];
+  out->pi2U1 = h.red[RED_PI2U1];
+  out->B2U1 = h.red[RED_B2U1];
+  out->reserved = 0.0;
+  return CLMI_OK;
+}
+
+int clmi_scale_factor_kernel(clmi_ctx *c, double *K)
+{
+  CHK(check_ctx(c));
+  if (!K) return fail(c, CLMI_E_INVALID, "null output");
+  CHK(flush(c));
+  CHK(update(c, 0, {{OP_KERNEL_ONLY, 0.0}}));
+  DevState h{};
+  CHK(read_state(c, &h));
+  *K = h.K;
+  return CLMI_OK;
+}
+
+int clmi_hubble_constraint(clmi_ctx *c, double out[3])
+{
+  CHK(clmi_set_all_averages(c));
+  DevState h{};
+  CHK(read_state(c, &h));
+  const clmi_background &b = h.bg;
+  const clmi_config &g = c->cfg;
+  const double a = b.aI;
+  double rho = b.potAvI;
+  if (g.n_scalars > 0) rho += 0.5 * b.pi2AvI * std::pow(a, -6) + 0.5 * b.grad2AvI * std::pow(a, -2);
+  if (g.n_cscalars > 0) rho += b.CSpi2AvI * std::pow(a, -6) + b.CSgrad2AvI * std::pow(a, -2);
+  if (g.n_u1 > 0) {
+    const double w2 = (g.omegaStar / g.fStar) * (g.omegaStar / g.fStar);
+    rho += 0.5 * w2 * std::pow(a, -4) * (b.U1pi2AvI + b.U1Mag2AvI);
+  }
+  const double lhs = b.aDotI * b.aDotI;
+  const double fm = g.fStar / g.MPl;
+  const double rhs = std::pow(a, 2 * g.alpha + 2) / 3.0 * (fm * fm) * rho;
+  out[0] = (lhs - rhs) / (lhs + rhs);
+  out[1] = lhs;
+  out[2] = rhs;
+  return CLMI_OK;
+}
+
+int clmi_gauss_u1(clmi_ctx *c, double out[3])
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  if (c->cfg.n_u1 < 1) return fail(c, CLMI_E_STATE, "model has no U(1) field");
+  CHK(flush(c));
+  GaussArgs a{};
+  for (int d = 0; d < 3; ++d) a.piA[d] = c->pU[d][c->cur_pG].p;
+  for (int k = 0; k < 2; ++k) {
+    a.phi[k] = c->fCS[k][c->cur_fG].p;
+    a.philo[k] = c->fCS[k][c->cur_fG].lo;
+    a.pi[k] = c->pCS[k][c->cur_pG].p;
+  }
+  a.N = c->N;
+  a.V = c->V;
+  a.f32 = c->f32;
+  a.has_cs = c->cfg.n_cscalars > 0;
+  a.inv_dx = 1.0 / c->cfg.dx;
+  a.dx = c->cfg.dx;
+  const double fw = c->cfg.fStar / c->cfg.omegaStar;
+  a.qnorm_gQ = 2.0 / c->cfg.dx * (fw * fw) * c->cfg.gQ;
+  a.partials = c->partials;
+  HIPCHK(c, launch_gauss(a, c->blocks_g, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, c->blocks_g, c->dscratch, c->stream));
+  double h[kNumRed];
+  HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double V = (double)c->V;
+  out[0] = (h[0] / V) / (h[1] / V);
+  out[1] = h[3] / V;
+  out[2] = h[2] / V;
+  return CLMI_OK;
+}
+
+int clmi_field_means(clmi_ctx *c, int kind, int index, int comp, double scale, double out[2])
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  CHK(flush(c));
+  DevArr *arr = nullptr;
+  bool is_field = false;
+  CHK(select_arr(c, kind, index, comp, &arr, &is_field));
+  MeansArgs a{};
+  a.p = arr->p;
+  a.lo = arr->lo;
+  a.V = c->V;
+  a.f32 = c->f32;
+  a.scale = scale;
+  a.partials = c->partials;
+  const int blocks = scalar_pass_blocks(c->V);
+  HIPCHK(c, launch_means(a, blocks, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, blocks, c->dscratch, c->stream));
+  double h[kNumRed];
+  HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = h[0] / (double)c->V;
+  out[1] = h[1] / (double)c->V;
+  return CLMI_OK;
+}
+
+int clmi_spectrum_nbins(const clmi_ctx *c, double dk)
+{
+  if (!c || !(dk > 0)) return CLMI_E_INVALID;
+  return (int)std::floor(std::sqrt(3.0) / 2.0 * c->N / dk) + 1;
+}
+
+int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, int nbins, double *sum_f2,
+                       double *count, double *mean_n)
+{
+  CHK(check_ctx(c));
+  if (!(dk > 0) || nbins != clmi_spectrum_nbins(c, dk) || !sum_f2)
+    return fail(c, CLMI_E_INVALID, "nbins must equal clmi_spectrum_nbins(ctx, dk)");
+  CHK(flush(c));
+  DevArr *arr = nullptr;
+  bool is_field = false;
+  CHK(select_arr(c, kind, index, comp, &arr, &is_field));
+  CHK(ensure_fft(c, true, false, false));
+  if (c->f32)
+    HIPCHK(c, launch_join_f64((const float *)arr->p, arr->lo, c->stage, c->V, c->stream));
+  else
+    HIPCHK(c, hipMemcpyAsync(c->stage, arr->p, sizeof(double) * c->V, hipMemcpyDeviceToDevice, c->stream));
+  FFTCHK(c, hipfftExecD2Z(c->plan_r2c, (hipfftDoubleReal *)c->stage, (hipfftDoubleComplex *)c->fk));
+  if (c->bins.nbins != nbins || c->bins.dk != dk || c->bins.N != c->N) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, build_bin_table(c->bins, c->N, dk, nbins));
+  }
+  if (c->bin_out_n < nbins) {
+    if (c->bin_out) HIPCHK(c, hipFree(c->bin_out));
+    HIPCHK(c, hipMalloc(&c->bin_out, sizeof(double) * nbins));
+    c->bin_out_n = nbins;
+  }
+  HIPCHK(c, launch_bin_sums(c->bins, c->fk, c->bin_out, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sum_f2, c->bin_out, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nbins; ++b) {
+    if (count) count[b] = c->bins.count[b];
+    if (mean_n) mean_n[b] = c->bins.mean_n[b];
+  }
+  return CLMI_OK;
+}
+
+int clmi_grf_scalar(clmi_ctx *c, int index, uint64_t seed, uint32_t stream, double mass2, double aDot, double kIR,
+                    double kCutOff, double fld0, double pi0)
+{
+  CHK(check_ctx(c));
+  if (index < 0 || index >= c->cfg.n_scalars) return fail(c, CLMI_E_INVALID, "no such scalar field");
+  if (!(kIR > 0)) return fail(c, CLMI_E_INVALID, "kIR must be > 0");
+  CHK(flush(c));
+  CHK(ensure_fft(c, false, true, true));
+  const int N = c->N;
+  const double dx = c->cfg.dx;
+  const double L = N * dx;
+  GrfArgs a{};
+  a.N = N;
+  a.nmodes = (uint32_t)N * N * (N / 2 + 1);
+  a.seed = seed;
+  a.stream = stream;
+  a.mass2 = mass2;
+  a.aDot = aDot;
+  a.kIR = kIR;
+  a.kCut = kCutOff;
+  a.sigma_pref = c->cfg.omegaStar / c->cfg.fStar * std::pow(L / (dx * dx), 1.5) / std::sqrt(2.0);
+  a.phik = c->fk;
+  a.pik = c->fk2;
+  // largest |n|^2 with sqrt(|n|^2) * kIR <= kCutOff
+  int64_t n2max;
+  const double r = kCutOff / kIR;
+  if (!(r >= 0))
+    n2max = -1;
+  else if (r > 4.0 * N)
+    n2max = (int64_t)3 * N * N;
+  else {
+    n2max = (int64_t)std::floor(r * r);
+    while (std::sqrt((double)(n2max + 1)) * kIR <= kCutOff) ++n2max;
+    while (n2max >= 0 && std::sqrt((double)n2max) * kIR > kCutOff) --n2max;
+  }
+  HIPCHK(c, launch_grf_modes(a, n2max, c->stream));
+  const double scale = 1.0 / ((double)N * N * N);
+  DevArr &f = c->fS[index][c->cur_fS];
+  DevArr &p = c->pS[index][c->cur_pS];
+  FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk, (hipfftDoubleReal *)c->stage));
+  HIPCHK(c, launch_grf_finish(c->stage, scale, fld0, f.p, f.lo, c->f32, c->V, c->stream));
+  FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk2, (hipfftDoubleReal *)c->stage));
+  HIPCHK(c, launch_grf_finish(c->stage, scale, pi0, p.p, nullptr, c->f32, c->V, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_profile_enable(clmi_ctx *c, int on)
+{
+  CHK(check_ctx(c));
+  CHK(prof_flush(c));
+  c->prof = on != 0;
+  c->prof_ms = 0.0;
+  c->prof_launches = 0;
+  return CLMI_OK;
+}
+
+int clmi_profile_read(clmi_ctx *c, double *total_ms, int64_t *launches, double *bytes_per_launch)
+{
+  CHK(check_ctx(c));
+  CHK(prof_flush(c));
+  if (total_ms) *total_ms = c->prof_ms;
+  if (launches) *launches = c->prof_launches;
+  if (bytes_per_launch) *bytes_per_launch = fused_pass_bytes(c);
+  return CLMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+// Internal declarations shared by the HIP translation units of libclmi.so.
+//
+// Design: every evolved lattice array lives in HBM for the lifetime of a context.  A leapfrog step
+// is ONE lattice pass (the deferred drift of the previous step fused with this step's kick and all
+// volume reductions), followed by a one-block "background update" kernel that finishes the
+// reductions in a fixed order and advances the scale factor on the device.  No host round trip
+// happens inside the time loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "clmi.h"
+
+namespace clmi {
+
+constexpr int kBlock = 256;        // threads per block of the lattice passes (4 wavefronts)
+constexpr int kNumRed = 8;         // reduction slots per block row
+constexpr int kMaxPassBlocks = 4096;
+
+// Reduction slots (volume sums; the update kernel divides by N^3).
+enum RedSlot : int {
+  RED_PI2S = 0,
+  RED_GRAD2S = 1,
+  RED_POT = 2,
+  RED_PI2CS = 3,
+  RED_GRAD2CS = 4,
+  RED_PI2U1 = 5,
+  RED_B2U1 = 6,
+  RED_SPARE = 7,
+};
+
+// Device-resident homogeneous state plus the per-pass coefficients derived from it.
+struct DevState {
+  clmi_background bg;
+  double K;         // last ScaleFactorKernels::get value
+  double aK;        // aI^(1+alpha)   -- Laplacian term of the scalar / complex-scalar kick
+  double aV;        // aI^(3+alpha)   -- potential term of the scalar / complex-scalar kick
+  double aG;        // aI^(alpha-1)   -- gradient term of the U(1) kick
+  double aJ;        // aI^(1+alpha)   -- current term of the U(1) kick
+  double fdrift_s;  // aSI^(alpha-3)  -- scalar / complex-scalar drift factor
+  double fdrift_u;  // aSI^(alpha-1)  -- U(1) drift factor
+  double red[kNumRed];  // last finished volume averages
+  double pad[3];
+};
+
+// Polynomial potential of the scalar singlets in program units.
+struct ScalarPoly {
+  int nterms;
+  int pad;
+  double c[CLMI_MAX_TERMS];
+  int e[CLMI_MAX_TERMS][CLMI_MAX_SCALARS];
+};
+
+// Complex-scalar potential V = sum_k c_k |phi|^(2 h_k).
+struct CsPoly {
+  int nterms;
+  int pad;
+  double c[CLMI_MAX_TERMS];
+  int h[CLMI_MAX_TERMS];  // half powers
+};
+
+// Model constants needed by the background update.
+struct ModelConsts {
+  int ns, ncs, nu1, expansion;
+  double dt, alpha, fStar, omegaStar, MPl;
+  double invV;
+};
+
+// Operations of the background update kernel, executed in order by one thread.
+enum UpdateOp : int {
+  OP_STORE_FIELDS = 1,   // storeFieldsAverages        (leapfrog.h:269-277, velocityverlet.h:258-266)
+  OP_LF_MOMENTA = 2,     // storeMomentaAverages + kickScaleFactor + driftScaleFactor (leapfrog.h:67-78)
+  OP_LF_SYNC = 3,        // sync's averages + aDotI    (leapfrog.h:105-112)
+  OP_LF_INIT = 4,        // aDotSI = aDotI - dt/2 K    (leapfrog.h:35)
+  OP_VV_SF_HALF = 5,     // kickScaleFactorHalf        (velocityverlet.h:165-168)
+  OP_VV_MOMENTA = 6,     // storeMomentaAverages       (velocityverlet.h:249-256)
+  OP_VV_DRIFT_SF = 7,    // driftScaleFactor           (velocityverlet.h:225-243)
+  OP_VV_SF_ONE = 8,      // kickScaleFactorOne         (velocityverlet.h:171-174)
+  OP_SET_ALL = 9,        // Averages::setAllAverages   (averages.h:86-129)
+  OP_KERNEL_ONLY = 10,   // recompute K only
+};
+
+struct UpdateOps {
+  int n;
+  int op[6];
+  int pad;
+  double w[6];
+};
+
+// One real lattice array in either precision.
+struct DevArr {
+  void *p = nullptr;   // double* (F64) or float* hi (F32 bulk)
+  float *lo = nullptr; // F32 bulk fields: low word; nullptr for momenta / F64
+};
+
+struct ScalarPassArgs {
+  const void *phi_in[CLMI_MAX_SCALARS];
+  const float *philo_in[CLMI_MAX_SCALARS];
+  const void *pi_in[CLMI_MAX_SCALARS];
+  void *phi_out[CLMI_MAX_SCALARS];
+  float *philo_out[CLMI_MAX_SCALARS];
+  void *pi_out[CLMI_MAX_SCALARS];
+  int N;
+  uint32_t V;
+  double inv_dx2;
+  double inv_dx;
+  double cdt;    // drift: phi += fdrift_s * (cdt * pi)
+  double ckick;  // kick: pi += ckick * (aK * lap - aV * dV)
+  const DevState *st;
+  double *partials;  // this pass's rows, kNumRed doubles each
+  ScalarPoly poly;
+};
+
+struct GaugePassArgs {
+  const void *phi_in[2];
+  const float *philo_in[2];
+  const void *pi_in[2];
+  const void *A_in[3];
+  const float *Alo_in[3];
+  const void *piA_in[3];
+  void *phi_out[2];
+  float *philo_out[2];
+  void *pi_out[2];
+  void *A_out[3];
+  float *Alo_out[3];
+  void *piA_out[3];
+  int N;
+  uint32_t V;
+  int has_u1;
+  int pad;
+  double dx, inv_dx, inv_dx2;
+  double gQ;
+  double jnorm;  // 2/dx * (fStar/omegaStar)^2
+  double cdt;
+  double ckick;
+  const DevState *st;
+  double *partials;
+  CsPoly poly;
+};
+
+// Deterministic block reduction: wavefront shuffles, then a fixed-order sum over the block's
+// wavefronts; thread k < K writes slot k of the block's row.
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
+{
+  __shared__ double sh[kBlock / 64][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += sh[w][threadIdx.x];
+    row[threadIdx.x] = s;
+  }
+}
+
+// Launchers (defined in the .hip files).
+int scalar_pass_blocks(uint32_t V);
+hipError_t launch_scalar_pass(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
+                              int blocks, hipStream_t s);
+int gauge_pass_blocks(uint32_t V);
+hipError_t launch_gauge_pass(bool f32, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks,
+                             hipStream_t s);
+hipError_t launch_update(DevState *st, const double *partials, int rows, const ModelConsts &mc,
+                         const UpdateOps &ops, hipStream_t s);
+
+struct GaussArgs {
+  void *piA[3];
+  void *phi[2];
+  float *philo[2];
+  void *pi[2];
+  int N;
+  uint32_t V;
+  int f32;
+  int has_cs;
+  double inv_dx, dx, qnorm_gQ;
+  double *partials;
+};
+hipError_t launch_gauss(const GaussArgs &a, int blocks, hipStream_t s);
+
+struct MeansArgs {
+  const void *p;
+  const float *lo;
+  uint32_t V;
+  int f32;
+  double scale;
+  double *partials;
+};
+hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s);
+hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s);
+
+// Array conversion kernels (upload / download in F32-bulk mode, FFT staging).
+hipError_t launch_split_f64(const double *src, float *hi, float *lo, uint32_t V, hipStream_t s);
+hipError_t launch_to_f32(const double *src, float *dst, uint32_t V, hipStream_t s);
+hipError_t launch_join_f64(const float *hi, const float *lo, double *dst, uint32_t V, hipStream_t s);
+
+// Spectrum binning and Gaussian random fields (fft.hip).
+struct BinTable {
+  int N = 0;
+  double dk = 0;
+  int nbins = 0;
+  uint32_t *d_modes = nullptr;   // half-space mode indices sorted by bin
+  uint32_t *d_start = nullptr;   // nbins + 1 offsets
+  std::vector<double> count, mean_n;
+};
+hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins);
+void free_bin_table(BinTable &t);
+hipError_t launch_bin_sums(const BinTable &t, const void *fk /*double2*/, double *out, hipStream_t s);
+
+struct GrfArgs {
+  int N;
+  uint32_t nmodes;
+  uint64_t seed;
+  uint32_t stream;
+  double mass2, aDot, kIR, kCut;
+  double sigma_pref;  // (omegaStar/fStar) * (L/dx^2)^(3/2) / sqrt(2)
+  void *phik;  // double2
+  void *pik;   // double2
+};
+hipError_t launch_grf_modes(const GrfArgs &a, int64_t n2max, hipStream_t s);
+hipError_t launch_grf_finish(const double *src, double scale, double add, void *dst, float *dstlo, int f32,
+                             uint32_t V, hipStream_t s);
+
+}  // namespace clmi

@@ -1,0 +1,252 @@
+// Kernels around the FFTs: |n|-shell binning of power spectra and Gaussian-random-field modes.
+//
+// Binning (measurements/powerspectrum.h:85-134 with TempLat's projectRadially): modes of the
+// half-complex R2C array are grouped by radial bin once per (N, dk) into a bin-sorted index table
+// (CSR).  A spectrum is then one block per bin summing w |f_k|^2 over its modes in a fixed order,
+// so bin sums are bit-reproducible (no floating-point atomics).  w = 1 on the kz = 0 and kz = N/2
+// planes and 2 elsewhere, so sums run over the full N^3 mode set.
+//
+// Gaussian random field (initializers/fluctuationsgenerator.h:39-71,103-150): per mode two complex
+// Gaussian amplitudes fL, fR of rms sigma_k per real part; phi_k = (fL+fR)/sqrt2 and
+// pi_k = i w_k (fL-fR)/sqrt2 - aDot phi_k.  Random numbers come from Philox4x32-10 (Salmon et al.,
+// SC'11) with key = seed and counter = (mode, draw, stream, 0), so values depend only on
+// (seed, stream, mode).  On the kz = 0 and kz = N/2 planes the array must be Hermitian: the mode
+// with the larger linear index of a conjugate pair takes its partner's draws with L and R swapped
+// and conjugated, and a self-conjugate mode uses fR = conj(fL); phi and pi are then exactly
+// Hermitian and real after the C2R transform.
+#include <algorithm>
+
+#include "clmi_internal.hpp"
+
+namespace clmi {
+
+hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins)
+{
+  free_bin_table(t);
+  const uint32_t Nh = N / 2 + 1;
+  const uint32_t M = (uint32_t)N * N * Nh;
+  std::vector<uint32_t> bin(M);
+  std::vector<uint32_t> cnt(nbins + 1, 0);
+  t.count.assign(nbins, 0.0);
+  t.mean_n.assign(nbins, 0.0);
+  std::vector<double> sumn(nbins, 0.0);
+  for (uint32_t kx = 0; kx < (uint32_t)N; ++kx) {
+    const int nx = kx <= (uint32_t)N / 2 ? (int)kx : (int)kx - N;
+    for (uint32_t ky = 0; ky < (uint32_t)N; ++ky) {
+      const int ny = ky <= (uint32_t)N / 2 ? (int)ky : (int)ky - N;
+      for (uint32_t kz = 0; kz < Nh; ++kz) {
+        const uint32_t m = (kx * N + ky) * Nh + kz;
+        const double nn = std::sqrt((double)(nx * nx + ny * ny + (int)(kz * kz)));
+        int b = (int)std::floor(nn / dk + 0.5);
+        if (b > nbins - 1) b = nbins - 1;
+        bin[m] = (uint32_t)b;
+        cnt[b + 1]++;
+        const double w = (kz == 0 || 2 * kz == (uint32_t)N) ? 1.0 : 2.0;
+        t.count[b] += w;
+        sumn[b] += w * nn;
+      }
+    }
+  }
+  for (int b = 0; b < nbins; ++b) {
+    t.mean_n[b] = t.count[b] > 0 ? sumn[b] / t.count[b] : 0.0;
+    cnt[b + 1] += cnt[b];
+  }
+  std::vector<uint32_t> start(cnt);  // offsets
+  std::vector<uint32_t> order(M);
+  std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (uint32_t m = 0; m < M; ++m) order[fill[bin[m]]++] = m;
+  hipError_t e = hipMalloc(&t.d_modes, sizeof(uint32_t) * (size_t)M);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(&t.d_start, sizeof(uint32_t) * (size_t)(nbins + 1));
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(t.d_modes, order.data(), sizeof(uint32_t) * (size_t)M, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(t.d_start, start.data(), sizeof(uint32_t) * (size_t)(nbins + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  t.N = N;
+  t.dk = dk;
+  t.nbins = nbins;
+  return hipSuccess;
+}
+
+void free_bin_table(BinTable &t)
+{
+  if (t.d_modes) (void)hipFree(t.d_modes);
+  if (t.d_start) (void)hipFree(t.d_start);
+  t.d_modes = nullptr;
+  t.d_start = nullptr;
+  t.N = 0;
+  t.nbins = 0;
+}
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_bin_sums(const uint32_t *modes, const uint32_t *start, const double2 *fk,
+                                                  uint32_t Nh, uint32_t Nz2, double *out)
+{
+  __shared__ double sh[256];
+  const uint32_t b = blockIdx.x;
+  const uint32_t lo = start[b], hi = start[b + 1];
+  double s = 0.0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+    const uint32_t m = modes[i];
+    const uint32_t kz = m % Nh;
+    const double2 v = fk[m];
+    const double w = (kz == 0 || 2 * kz == Nz2) ? 1.0 : 2.0;
+    s += w * (v.x * v.x + v.y * v.y);
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride) sh[threadIdx.x] += sh[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = sh[0];
+}
+
+// ---- Philox4x32-10 ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const uint64_t p0 = (uint64_t)M0 * c.v[0];
+    const uint64_t p1 = (uint64_t)M1 * c.v[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    U4 n;
+    n.v[0] = hi1 ^ c.v[1] ^ k0;
+    n.v[1] = lo1;
+    n.v[2] = hi0 ^ c.v[3] ^ k1;
+    n.v[3] = lo0;
+    c = n;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b)
+{
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Two standard normals (Box-Muller on one uniform pair) for (mode, draw).
+__device__ __forceinline__ void normals2(uint64_t seed, uint32_t stream, uint32_t m, uint32_t draw, double g[2])
+{
+  U4 c;
+  c.v[0] = m;
+  c.v[1] = draw;
+  c.v[2] = stream;
+  c.v[3] = 0u;
+  const U4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double u1 = u53(r.v[0], r.v[1]), u2 = u53(r.v[2], r.v[3]);
+  const double rad = sqrt(-2.0 * log(1.0 - u1));
+  double sn, cs;
+  sincos(6.283185307179586477 * u2, &sn, &cs);
+  g[0] = rad * cs;
+  g[1] = rad * sn;
+}
+
+__global__ __launch_bounds__(256) void k_grf_modes(const GrfArgs a, int64_t n2max)
+{
+  const uint32_t N = a.N, Nh = N / 2 + 1;
+  for (uint32_t m = blockIdx.x * 256 + threadIdx.x; m < a.nmodes; m += gridDim.x * 256) {
+    const uint32_t kz = m % Nh;
+    const uint32_t kxy = m / Nh;
+    const uint32_t kx = kxy / N, ky = kxy - kx * N;
+    const int nx = kx <= N / 2 ? (int)kx : (int)kx - (int)N;
+    const int ny = ky <= N / 2 ? (int)ky : (int)ky - (int)N;
+    const int64_t n2 = (int64_t)nx * nx + (int64_t)ny * ny + (int64_t)kz * kz;
+    double2 ph = make_double2(0.0, 0.0), pp = make_double2(0.0, 0.0);
+    if (m != 0 && n2 <= n2max) {
+      const double k = sqrt((double)n2) * a.kIR;
+      const double meff2 = a.mass2 > 0 ? a.mass2 : 0.0;
+      const double omega = sqrt(k * k + meff2);
+      const double sigma = a.sigma_pref * pow(2 * omega, -0.5);
+      // canonical representative of the conjugate pair on the kz = 0 and kz = N/2 planes
+      uint32_t src = m;
+      int mode = 0;  // 0: own draws, 1: partner's draws swapped+conjugated, 2: self-conjugate
+      if (kz == 0 || 2 * kz == N) {
+        const uint32_t px = (N - kx) % N, py = (N - ky) % N;
+        const uint32_t pm = (px * N + py) * Nh + kz;
+        if (pm == m)
+          mode = 2;
+        else if (pm < m) {
+          mode = 1;
+          src = pm;
+        }
+      }
+      double gL[2], gR[2];
+      normals2(a.seed, a.stream, src, 0u, gL);
+      normals2(a.seed, a.stream, src, 1u, gR);
+      double2 fL = make_double2(sigma * gL[0], sigma * gL[1]);
+      double2 fR = make_double2(sigma * gR[0], sigma * gR[1]);
+      if (mode == 1) {
+        const double2 tL = fL;
+        fL = make_double2(fR.x, -fR.y);
+        fR = make_double2(tL.x, -tL.y);
+      } else if (mode == 2) {
+        fR = make_double2(fL.x, -fL.y);
+      }
+      const double is2 = 0.70710678118654752440;
+      ph = make_double2((fL.x + fR.x) * is2, (fL.y + fR.y) * is2);
+      // i * omega * (fL - fR) / sqrt2 - aDot * phi
+      const double dr = (fL.x - fR.x) * is2, di = (fL.y - fR.y) * is2;
+      pp = make_double2(-omega * di - a.aDot * ph.x, omega * dr - a.aDot * ph.y);
+    }
+    ((double2 *)a.phik)[m] = ph;
+    ((double2 *)a.pik)[m] = pp;
+  }
+}
+
+__global__ void k_grf_finish(const double *src, double scale, double add, void *dst, float *dstlo, int f32,
+                             uint32_t V)
+{
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    const double v = src[i] * scale + add;
+    if (f32) {
+      const float h = (float)v;
+      ((float *)dst)[i] = h;
+      if (dstlo) dstlo[i] = (float)(v - (double)h);
+    } else {
+      ((double *)dst)[i] = v;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_bin_sums(const BinTable &t, const void *fk, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bin_sums, dim3(t.nbins), dim3(256), 0, s, t.d_modes, t.d_start, (const double2 *)fk,
+                     (uint32_t)(t.N / 2 + 1), (uint32_t)t.N, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_grf_modes(const GrfArgs &a, int64_t n2max, hipStream_t s)
+{
+  uint32_t blocks = (a.nmodes + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_grf_modes, dim3(blocks), dim3(256), 0, s, a, n2max);
+  return hipGetLastError();
+}
+
+hipError_t launch_grf_finish(const double *src, double scale, double add, void *dst, float *dstlo, int f32,
+                             uint32_t V, hipStream_t s)
+{
+  uint32_t blocks = (V + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_grf_finish, dim3(blocks), dim3(256), 0, s, src, scale, add, dst, dstlo, f32, V);
+  return hipGetLastError();
+}
+
+}  // namespace clmi

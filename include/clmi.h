@@ -1,0 +1,200 @@
+/*
+ * clmi.h -- C ABI of the MI355X-native CosmoLattice lattice-evolution library (libclmi.so).
+ *
+ * The reference evaluates its evolution as C++ templates (CosmoInterface over TempLat); there is no
+ * C ABI in the reference.  Each entry point below replaces one reference interface, cited as
+ * path:line relative to the CosmoLattice source tree.  No C++ or torch types cross this boundary:
+ * plain pointers, sizes and POD structs only.  Every function returns CLMI_OK (0) on success or a
+ * negative CLMI_E_* code; clmi_last_error() then holds a message.  No exception crosses the ABI.
+ *
+ * Data format at the boundary: every real lattice array is N*N*N doubles, dense row-major
+ * [x][y][z] with z fastest, no ghost layers, periodic.  Direction i = 1, 2, 3 of the reference is
+ * x, y, z here.  Host buffers are borrowed for the duration of the call only.
+ *
+ * Threading: one host thread drives a context.  All work is ordered on the context's own HIP
+ * stream.  Calls on the same context are not re-entrant; several contexts may coexist.
+ */
+#ifndef CLMI_H
+#define CLMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLMI_OK 0
+#define CLMI_E_INVALID (-1)   /* bad argument or inconsistent configuration */
+#define CLMI_E_HIP (-2)       /* a HIP runtime / hipFFT call failed */
+#define CLMI_E_STATE (-3)     /* call not valid in the context's current state */
+#define CLMI_E_NOMEM (-4)     /* device allocation failed */
+
+#define CLMI_MAX_SCALARS 4
+#define CLMI_MAX_TERMS 8
+
+/* Device precision modes.  CLMI_F64: every array and every operation in FP64 (the drop-in mode).
+ * CLMI_F32_BULK: fields stored as FP32 hi/lo pairs, momenta FP32, stencil/kick/drift in FP32;
+ * volume reductions, scale-factor update, spectra bin sums and constraint checks in FP64. */
+#define CLMI_F64 0
+#define CLMI_F32_BULK 1
+
+/* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
+#define CLMI_FLD_S 0
+#define CLMI_PI_S 1
+#define CLMI_FLD_CS 2
+#define CLMI_PI_CS 3
+#define CLMI_FLD_U1 6
+#define CLMI_PI_U1 7
+
+/* Model and run configuration.  Replaces the model/run state the reference builds in
+ * AbstractModel (abstractmodel.h:147-186), RunParameters (runparameters.h:60-236) and the
+ * model's potentialTerms/potDeriv (models/lphi4.h:154-206, models/lphi4U1.h:117-129).
+ *
+ * The scalar-singlet potential is a polynomial in program units,
+ *   V(phi) = sum_t term_coeff[t] * prod_f phi_f^term_exp[t][f],
+ * and the complex-scalar potential is V(|phi|) = sum_k cs_coeff[k] * |phi|^cs_pow[k] (even powers).
+ * The library differentiates both analytically. */
+typedef struct clmi_config {
+  int N;          /* sites per side */
+  int n_scalars;  /* real scalar singlets, 0..CLMI_MAX_SCALARS */
+  int n_cscalars; /* complex scalars, 0 or 1 */
+  int n_u1;       /* U(1) gauge fields, 0 or 1 (requires n_cscalars == 1) */
+  int precision;  /* CLMI_F64 or CLMI_F32_BULK */
+  int expansion;  /* 1: self-consistent expansion (runparameters.h:66); 0: static a = aI */
+  int device;     /* HIP device ordinal */
+  int reserved;
+  double dx, dt;
+  double alpha, fStar, omegaStar, MPl; /* program units (modelparametersbase.h:20, scalefactorbase.h:18) */
+  double gQ;                           /* effective U(1) charge g*Q (couplingsmanager.h:70-94) */
+  int n_terms;
+  int n_cs_terms;
+  double term_coeff[CLMI_MAX_TERMS];
+  int term_exp[CLMI_MAX_TERMS][CLMI_MAX_SCALARS];
+  double cs_coeff[CLMI_MAX_TERMS];
+  int cs_pow[CLMI_MAX_TERMS];
+} clmi_config;
+
+/* Homogeneous state: scale factor and cached volume averages, named as in the reference's
+ * ScaleFactorBase (abstractmodel/scalefactorbase.h:20-21), ScalarBase, ComplexScalarBase
+ * (abstractmodel/complexscalarbase.h:33-34), U1Base and AbstractModel (abstractmodel.h:142). */
+typedef struct clmi_background {
+  double aI, aSI, aIM, aDotI, aDotSI, aDotSIM;
+  double pi2AvI, pi2AvSI, pi2AvSIM, grad2AvI;
+  double CSpi2AvI, CSpi2AvSI, CSpi2AvSIM, CSgrad2AvI;
+  double U1pi2AvI, U1pi2AvSI, U1pi2AvSIM, U1Mag2AvI;
+  double potAvI, potAvSI;
+} clmi_background;
+
+/* Volume averages of the current device fields (no state change).  Replaces
+ * Averages::pi2S/grad2S/pi2CS/grad2CS/B2U1/pi2U1 (definitions/averages.h:27-71) and
+ * average(Potential::potential) (evolvers/leapfrog.h:276). */
+typedef struct clmi_averages {
+  double pi2S, grad2S, pot, pi2CS, grad2CS, pi2U1, B2U1;
+  double reserved;
+} clmi_averages;
+
+typedef struct clmi_ctx clmi_ctx;
+
+/* ---- lifetime ------------------------------------------------------------------------------ */
+
+/* Allocate all device storage for the model (fields zeroed, background a = 1).  Replaces the
+ * model constructor's field allocation (abstractmodel.h:147-156, abstractmodel/scalarbase.h:81-88). */
+int clmi_create(const clmi_config *cfg, clmi_ctx **out);
+void clmi_destroy(clmi_ctx *ctx);
+/* Message of the last failed call on ctx, or of the last failed clmi_create when ctx is NULL. */
+const char *clmi_last_error(const clmi_ctx *ctx);
+/* Block until all work queued on the context's stream has finished. */
+int clmi_synchronize(clmi_ctx *ctx);
+
+/* ---- data movement -------------------------------------------------------------------------- */
+
+/* Copy one real component array (N^3 doubles) to / from the device.  kind is a CLMI_FLD_* /
+ * CLMI_PI_* value, index the field number, comp the component (0..1 complex, 0..2 gauge vector).
+ * Replaces direct Field access in the reference's initializers and measurers.  download flushes a
+ * deferred drift first, so the values are at the evolver's current time. */
+int clmi_upload_field(clmi_ctx *ctx, int kind, int index, int comp, const double *host);
+int clmi_download_field(clmi_ctx *ctx, int kind, int index, int comp, double *host);
+
+/* Background scalars in and out (synchronous).  Replaces direct access to model.aI, model.aDotI,
+ * model.pi2AvI, ... (abstractmodel/scalefactorbase.h:20-21). */
+int clmi_set_background(clmi_ctx *ctx, const clmi_background *bg);
+int clmi_get_background(clmi_ctx *ctx, clmi_background *bg);
+
+/* ---- evolvers ----------------------------------------------------------------------------- */
+
+/* LeapFrog constructor for a new run: synced = true, aDotSI = aDotI - dt/2 * K
+ * (evolvers/leapfrog.h:29-38).  restart != 0 starts unsynced and leaves aDotSI alone. */
+int clmi_lf_init(clmi_ctx *ctx, int restart);
+/* nsteps x LeapFrog::evolve (evolvers/leapfrog.h:42-89).  The drift of the last step is kept
+ * deferred and fused into the next step's kick; every observable call flushes it. */
+int clmi_lf_evolve(clmi_ctx *ctx, int nsteps);
+/* LeapFrog::sync (evolvers/leapfrog.h:96-116): half-kick momenta to integer time. */
+int clmi_lf_sync(clmi_ctx *ctx);
+/* Whether momenta are currently synced with the fields (leapfrog.h:284). */
+int clmi_lf_is_synced(const clmi_ctx *ctx);
+/* nsteps x VelocityVerlet::evolve with composition weights ws[0..nws-1]
+ * (evolvers/velocityverlet.h:51-114; VV2 = {1.0}, velocityverletparameters.h:26-27). */
+int clmi_vv_evolve(clmi_ctx *ctx, const double *ws, int nws, int nsteps);
+
+/* ---- primitives (one lattice pass each) ----------------------------------------------------- */
+
+/* All momenta += c * kernel(fields) at the current aI (LeapFrog::kickScalar/kickCS/kickU1Vector,
+ * evolvers/leapfrog.h:131-155, with kernels evolvers/kernels/scalarsingletkernels.h:30-40,
+ * complexscalarkernels.h:28-35, u1kernels.h:31-52). */
+int clmi_kick(clmi_ctx *ctx, double c);
+/* All fields += aSI^(alpha-3) * cdt * pi (scalars, complex scalar) and aSI^(alpha-1) * cdt * piU1
+ * (LeapFrog::driftScalar/driftCS/driftU1Vector, evolvers/leapfrog.h:185-217). */
+int clmi_drift(clmi_ctx *ctx, double cdt);
+/* Averages::setAllAverages (definitions/averages.h:86-129), including its overwrite of the
+ * half-step caches pi2AvSI, potAvSI, CSpi2AvSI, U1pi2AvSI. */
+int clmi_set_all_averages(clmi_ctx *ctx);
+/* Volume averages of the current fields; no background change. */
+int clmi_volume_averages(clmi_ctx *ctx, clmi_averages *out);
+/* ScaleFactorKernels::get from the current background (evolvers/kernels/scalefactorkernels.h:27-81). */
+int clmi_scale_factor_kernel(clmi_ctx *ctx, double *K);
+/* HubbleConstraint::get (definitions/hubbleconstraint.h:27-36), which calls Energies::rho and so
+ * setAllAverages: out = {(LHS-RHS)/(LHS+RHS), LHS, RHS}. */
+int clmi_hubble_constraint(clmi_ctx *ctx, double out[3]);
+/* GaussLaws::checkU1 (definitions/gausslaws.h:31-53): out = {<|LHS-RHS|>/<sum of squares>, <|LHS|>,
+ * <|RHS|>}. */
+int clmi_gauss_u1(clmi_ctx *ctx, double out[3]);
+/* MeansMeasurer::measure inputs (measurements/meansmeasurer.h:28-48) for one component, with the
+ * array multiplied by scale: out = {<f>, <f^2>}. */
+int clmi_field_means(clmi_ctx *ctx, int kind, int index, int comp, double scale, double out[2]);
+
+/* ---- spectra and initial conditions ---------------------------------------------------------- */
+
+/* Number of radial bins used by clmi_spectrum_bins for bin width dk: floor(sqrt(3)/2*N/dk) + 1
+ * (runparameters.h:97-98 gives the reference's floor(sqrt(3)/2*N/dk); one more bin holds k = 0). */
+int clmi_spectrum_nbins(const clmi_ctx *ctx, double dk);
+/* R2C FFT of one component (unnormalised forward transform) and deterministic |n|-shell binning
+ * (measurements/powerspectrum.h:85-134 with TempLat's projectRadially).  Mode n (integer vector,
+ * components in (-N/2, N/2]) goes to bin b = min(floor(|n|/dk + 1/2), nbins-1).  Outputs, each of
+ * length nbins: sum over the full N^3 mode set of |f_n|^2, number of modes, and the mean |n| of
+ * the modes in the bin (0 for an empty bin). */
+int clmi_spectrum_bins(clmi_ctx *ctx, int kind, int index, int comp, double dk, int nbins, double *sum_f2,
+                       double *count, double *mean_n);
+
+/* Gaussian random field initial conditions for scalar singlet `index` on the device
+ * (initializers/fluctuationsgenerator.h:39-71,103-150, scalarsingletinitializer.h:41-77,283-284):
+ * vacuum modes with frequency sqrt(k^2 + max(mass2, 0)), cut at k > kCutOff, zero mode zero,
+ * pi_k = i w (fL - fR)/sqrt2 - aDot phi_k, then the homogeneous values fld0 and pi0 (program units)
+ * are added.  Random numbers: Philox4x32-10 keyed by (seed, stream) with counter (mode, draw), so
+ * the result does not depend on launch shape. */
+int clmi_grf_scalar(clmi_ctx *ctx, int index, uint64_t seed, uint32_t stream, double mass2, double aDot,
+                    double kIR, double kCutOff, double fld0, double pi0);
+
+/* ---- timing --------------------------------------------------------------------------------- */
+
+/* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */
+int clmi_profile_enable(clmi_ctx *ctx, int on);
+/* Total milliseconds and number of timed launches since enabling; bytes_per_launch is the
+ * algorithmic HBM traffic of one such pass (every evolved array read once and written once). */
+int clmi_profile_read(clmi_ctx *ctx, double *total_ms, int64_t *launches, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLMI_H */

@@ -1,0 +1,333 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on identical inputs.
+
+Tolerances: FP64 mode must match the oracle within 1e-10 relative on fields and volume averages
+(the bar stated for the FP64 drop-in); spectra within 1e-8.  FP32-bulk mode is checked against the
+same FP64 oracle with the looser bounds written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cosmolattice_amd.clmi import (CLMI_F32_BULK, CLMI_F64, FLD_CS, FLD_S, FLD_U1, PI_CS, PI_S, PI_U1,
+                                   Lattice, ModelSpec, lphi4, lphi4U1)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def smooth_random(N, rng, kmax=3, amp=0.1, waves=6):
+    x = np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((N, N, N))
+    for _ in range(waves):
+        n = rng.integers(-kmax, kmax + 1, size=3)
+        f += amp * rng.uniform(0.2, 1.0) * np.cos(2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / N
+                                                  + rng.uniform(0, 2 * np.pi))
+    return f
+
+
+def bench_spec(q=100.0, n_fields=2):
+    # lambda = 1, phi0 = 1 in program units; fStar / MPl = 2.34 so the expansion is live
+    return lphi4(q, n_fields=n_fields, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+
+
+def scalar_ics(spec, N, rng, noise=0.02):
+    out = {}
+    for f in range(spec.n_scalars):
+        out[(FLD_S, f)] = spec.fld0[f] + smooth_random(N, rng, amp=0.08) + noise * rng.normal(size=(N, N, N))
+        out[(PI_S, f)] = spec.pi0[f] + smooth_random(N, rng, amp=0.08) + noise * rng.normal(size=(N, N, N))
+    return out
+
+
+def load_both(lat, orc, ics):
+    for (k, i), a in ics.items():
+        lat.upload(k, i, 0, a)
+        orc.upload(k, i, 0, a)
+
+
+def init_background(lat, orc):
+    """aDotI from the Friedmann constraint on the loaded fields (modelinitializer.h:71-80)."""
+    orc.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=1.0)
+    _, _, rhs = orc.hubble_constraint()
+    orc.set_background(aDotI=math.sqrt(rhs))
+    lat.set_background(**orc.background())
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def run_reference_loop(obj, steps, out_every, evolver="LF", ws=(1.0,)):
+    recs = []
+    if evolver == "LF":
+        obj.lf_init()
+    for i in range(steps):
+        if i % out_every == 0:
+            if evolver == "LF":
+                obj.lf_sync()
+            recs.append(obj.hubble_constraint())
+        if evolver == "LF":
+            obj.lf_evolve(1)
+        else:
+            obj.vv_evolve(list(ws), 1)
+    if evolver == "LF":
+        obj.lf_sync()
+    recs.append(obj.hubble_constraint())
+    return np.array(recs)
+
+
+@pytest.mark.parametrize("N", [16, 32, 20])
+def test_lf_lphi4_fp64_parity_100_steps(gpu, oracle_mod, N):
+    """2-field lphi4, self-consistent expansion, leapfrog, 100 steps with measurements (sync +
+    Friedmann check) every 10 steps: fields, momenta, background and Friedmann records match the
+    oracle within 1e-10 relative."""
+    say(f"case lf_lphi4_fp64 N={N}")
+    spec = bench_spec()
+    dx, dt = 0.4909, 0.005
+    rng = np.random.default_rng(100 + N)
+    ics = scalar_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, dt, CLMI_F64, expansion=True)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=True)
+    load_both(lat, orc, ics)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 100, 10)
+    ro = run_reference_loop(orc, 100, 10)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        e = relerr(lat.download(*k, 0), orc.download(*k))
+        say(k, e)
+        assert e < FP64_TOL, k
+    bg, bo = lat.background(), orc.background()
+    for key in bo:
+        if bo[key] != 0:
+            assert abs(bg[key] - bo[key]) <= FP64_TOL * abs(bo[key]), key
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+    np.testing.assert_allclose(rg[:, 0], ro[:, 0], rtol=1e-6, atol=1e-13)
+    assert bg["aI"] > 1.0
+
+
+def test_vv2_single_field_fp64_parity(gpu, oracle_mod):
+    """Single-inflaton lphi4 with velocity Verlet (VV2), expansion on: 60 steps."""
+    say("case vv2_1field_fp64")
+    spec = bench_spec(n_fields=1)
+    N, dx, dt = 24, 0.98, 0.01
+    rng = np.random.default_rng(7)
+    ics = scalar_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, dt, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_both(lat, orc, ics)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 60, 20, evolver="VV")
+    ro = run_reference_loop(orc, 60, 20, evolver="VV")
+    for k in [(FLD_S, 0), (PI_S, 0)]:
+        assert relerr(lat.download(*k, 0), orc.download(*k)) < FP64_TOL
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+
+
+def test_primitives_fp64_parity(gpu, oracle_mod):
+    """kick, drift and every volume average as single passes (no fusion involved)."""
+    say("case primitives_fp64")
+    spec = bench_spec()
+    N, dx = 16, 0.7
+    rng = np.random.default_rng(9)
+    ics = scalar_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, 0.01, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, 0.01)
+    load_both(lat, orc, ics)
+    lat.set_background(aI=1.3, aSI=1.2)
+    orc.set_background(aI=1.3, aSI=1.2)
+    ag, ao = lat.averages(), orc.averages()
+    for k in ("pi2S", "grad2S", "pot"):
+        assert ag[k] == pytest.approx(ao[k], rel=1e-13)
+    lat.kick(0.013)
+    orc.kick(0.013)
+    lat.drift(0.021)
+    orc.drift(0.021)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        assert relerr(lat.download(*k, 0), orc.download(*k)) < 1e-14
+    assert lat.scale_factor_kernel() == pytest.approx(orc.scale_factor_kernel(), rel=1e-13)
+
+
+def test_free_plane_wave_closed_form_256(gpu):
+    """Full-size check at 256^3: a free massive plane wave follows phi_n = A cos(n W dt) exactly
+    (size-independent property of the leapfrog recurrence), in FP64 and FP32-bulk."""
+    say("case free_plane_wave_256")
+    N, dx, dt, m2 = 256, 0.4909, 0.005, 3.0
+    spec = ModelSpec(name="free", n_scalars=1, terms=[(0.5 * m2, [2])])
+    n = (5, 17, 3)
+    A = 0.5
+    x = np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    shape = np.cos(2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / N)
+    w2 = m2 + (4 / dx ** 2) * sum(math.sin(math.pi * ni / N) ** 2 for ni in n)
+    W = math.acos(1 - w2 * dt * dt / 2) / dt
+    steps = 200
+    for prec, tol in ((CLMI_F64, 1e-12), (CLMI_F32_BULK, 2e-5)):
+        lat = Lattice(spec, N, dx, dt, prec, expansion=False)
+        lat.upload(FLD_S, 0, 0, A * shape)
+        lat.lf_init()
+        lat.lf_evolve(steps)
+        lat.lf_sync()
+        phi = lat.download(FLD_S, 0, 0)
+        err = np.max(np.abs(phi - A * math.cos(steps * W * dt) * shape))
+        say("prec", prec, "err", err)
+        assert err < tol * A
+        lat.close()
+
+
+def test_f32_bulk_tracks_fp64(gpu, oracle_mod):
+    """FP32-bulk (hi/lo fields, FP32 momenta, FP64 reductions) stays close to the FP64 oracle over
+    200 leapfrog steps: fields within 1e-5 of their scale, Friedmann violation comparable."""
+    say("case f32_bulk_vs_fp64")
+    spec = bench_spec()
+    N, dx, dt = 32, 0.4909, 0.005
+    rng = np.random.default_rng(21)
+    ics = scalar_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, dt, CLMI_F32_BULK)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_both(lat, orc, ics)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 200, 50)
+    ro = run_reference_loop(orc, 200, 50)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        e = relerr(lat.download(*k, 0), orc.download(*k))
+        say(k, e)
+        assert e < 1e-5
+    assert np.max(np.abs(rg[:, 0] - ro[:, 0])) < 1e-6
+    assert lat.background()["aI"] == pytest.approx(orc.background()["aI"], rel=1e-8)
+
+
+def test_deterministic_reductions(gpu):
+    """Two identical runs give bit-identical fields and background (fixed-order reductions)."""
+    say("case determinism")
+    spec = bench_spec()
+    N = 32
+    rng = np.random.default_rng(5)
+    ics = scalar_ics(spec, N, rng)
+    outs = []
+    for _ in range(2):
+        lat = Lattice(spec, N, 0.4909, 0.005, CLMI_F64)
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.set_background(aDotI=0.5)
+        lat.lf_init()
+        lat.lf_evolve(30)
+        outs.append((lat.download(FLD_S, 1, 0), lat.background()))
+        lat.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
+def u1_spec():
+    spec = lphi4U1(e2_over_lambda=10.0, lam=1.0)
+    return spec
+
+
+def u1_ics(spec, N, rng, gauss_consistent=False):
+    f = {}
+    for c in range(2):
+        f[(FLD_CS, c)] = spec.fld0[c] + smooth_random(N, rng, amp=0.1)
+        f[(PI_CS, c)] = spec.pi0[c] + smooth_random(N, rng, amp=0.1)
+    for d in range(3):
+        f[(FLD_U1, d)] = smooth_random(N, rng, amp=0.05)
+        f[(PI_U1, d)] = smooth_random(N, rng, amp=0.05)
+    if gauss_consistent:
+        s = smooth_random(N, rng, amp=0.2)
+        f[(PI_CS, 0)] = s * f[(FLD_CS, 0)]
+        f[(PI_CS, 1)] = s * f[(FLD_CS, 1)]
+        for d in range(3):
+            f[(PI_U1, d)] = np.zeros((N, N, N))
+    return f
+
+
+def load_u1(lat, st):
+    for (k, c), a in st.items():
+        lat.upload(k, 0, c, a)
+
+
+def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod):
+    """Charged scalar + U(1), expansion on, 40 leapfrog steps with measurements every 10: all ten
+    arrays and the background match the oracle within 1e-10; Gauss checks agree."""
+    say("case lf_lphi4U1_fp64")
+    spec = u1_spec()
+    N, dx, dt = 16, 0.6, 0.01
+    rng = np.random.default_rng(31)
+    st = u1_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, dt, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_u1(lat, st)
+    load_u1(orc, st)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 40, 10)
+    ro = run_reference_loop(orc, 40, 10)
+    for (k, c) in st:
+        e = relerr(lat.download(k, 0, c), orc.download(k, 0, c))
+        say((k, c), e)
+        assert e < FP64_TOL, (k, c)
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+    gg, go = lat.gauss_u1(), orc.gauss_u1()
+    np.testing.assert_allclose(gg, go, rtol=1e-9)
+
+
+def test_lphi4u1_gauss_conserved_fp64_and_f32(gpu):
+    """Gauss-consistent initial data keep the Gauss violation at rounding level on the GPU."""
+    say("case u1_gauss")
+    spec = u1_spec()
+    N, dx, dt = 32, 0.6, 0.01
+    rng = np.random.default_rng(33)
+    st = u1_ics(spec, N, rng, gauss_consistent=True)
+    for prec, tol in ((CLMI_F64, 1e-11), (CLMI_F32_BULK, 1e-4)):
+        lat = Lattice(spec, N, dx, dt, prec, expansion=False)
+        load_u1(lat, st)
+        lat.lf_init()
+        lat.lf_evolve(50)
+        lat.lf_sync()
+        diff, lhs, rhs = lat.gauss_u1()
+        say("prec", prec, diff, lhs, rhs)
+        assert lhs > 1e-4
+        assert diff < tol
+        lat.close()
+
+
+def test_spectrum_bins_parity(gpu, oracle_mod):
+    """hipFFT R2C + deterministic shell binning equals the numpy restatement within 1e-12."""
+    say("case spectrum")
+    spec = bench_spec()
+    for N, dk in ((32, 1.0), (24, 0.5)):
+        rng = np.random.default_rng(N)
+        f = 1.0 + smooth_random(N, rng) + 0.1 * rng.normal(size=(N, N, N))
+        for prec in (CLMI_F64, CLMI_F32_BULK):
+            lat = Lattice(spec, N, 0.5, 0.01, prec)
+            lat.upload(FLD_S, 1, 0, f)
+            s, c, m = lat.spectrum_bins(FLD_S, 1, 0, dk)
+            so, co, mo = oracle_mod.spectrum_bins(lat.download(FLD_S, 1, 0), dk)
+            np.testing.assert_array_equal(c, co)
+            np.testing.assert_allclose(m, mo, rtol=1e-13)
+            np.testing.assert_allclose(s, so, rtol=1e-12, atol=1e-12 * so.max())
+            lat.close()
+
+
+def test_grf_parity(gpu, oracle_mod):
+    """Gaussian-random-field initial conditions on the GPU equal the numpy restatement (same
+    Philox stream) to rounding, in both precisions."""
+    say("case grf")
+    spec = bench_spec()
+    N, kIR = 32, 0.4
+    dx = 2 * math.pi / kIR / N
+    m2 = spec.mass2([1.0, 0.0])
+    for prec, tol in ((CLMI_F64, 1e-12), (CLMI_F32_BULK, 1e-5)):
+        lat = Lattice(spec, N, dx, 0.005, prec)
+        for f in range(2):
+            lat.grf_scalar(f, 12345, f, m2[f], 0.3, kIR, N * kIR / 4, spec.fld0[f], spec.pi0[f])
+            phi_o, pi_o = oracle_mod.grf_scalar(N, dx, spec, 12345, f, m2[f], 0.3, kIR, N * kIR / 4,
+                                                spec.fld0[f], spec.pi0[f])
+            ef = np.max(np.abs(lat.download(FLD_S, f, 0) - phi_o)) / np.max(np.abs(phi_o - spec.fld0[f]))
+            ep = np.max(np.abs(lat.download(PI_S, f, 0) - pi_o)) / np.max(np.abs(pi_o - spec.pi0[f]))
+            say("grf", prec, f, ef, ep)
+            assert ef < tol and ep < tol
+        lat.close()

@@ -1,0 +1,467 @@
+"""Pins of the FP64 CPU oracle against what the mathematics fixes.
+
+The reference ships no numeric golden vectors for this path (its tests cover coupling
+arithmetic only), so each oracle function is checked against closed forms, exact discrete
+recurrences, conservation laws and symmetries chosen so that a dropped term, a wrong sign or index,
+or a transposed operand fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cosmolattice_amd.clmi import ModelSpec, lphi4, lphi4U1
+
+FLD_S, PI_S, FLD_CS, PI_CS, FLD_U1, PI_U1 = 0, 1, 2, 3, 6, 7
+
+
+def free_field(m2: float) -> ModelSpec:
+    return ModelSpec(name="free", n_scalars=1, terms=[(0.5 * m2, [2])], fld0=[0.0], pi0=[0.0])
+
+
+def plane_wave(N, n, amp=1.0, phase=0.0):
+    x = np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    return amp * np.cos(2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / N + phase)
+
+
+def smooth_random(N, rng, kmax=3, amp=0.1):
+    f = np.zeros((N, N, N))
+    for _ in range(6):
+        n = rng.integers(-kmax, kmax + 1, size=3)
+        f += plane_wave(N, n, amp * rng.uniform(0.2, 1.0), rng.uniform(0, 2 * np.pi))
+    return f
+
+
+# ------------------------------------------------------------------------------------------------
+# RNG
+# ------------------------------------------------------------------------------------------------
+
+def test_philox_known_answers(oracle_mod):
+    """Philox4x32-10 known-answer vectors published with Random123 (kat_vectors)."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        got = oracle_mod.philox4x32_10(*ctr, *key)
+        assert tuple(int(g) for g in got) == want
+
+
+# ------------------------------------------------------------------------------------------------
+# Laplacian and potential
+# ------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("N,n", [(8, (1, 2, 3)), (12, (5, 0, 1)), (10, (0, 3, 4))])
+def test_laplacian_plane_wave_eigenvalue(oracle_mod, N, n):
+    """L e^{ik.x} = -(4/dx^2) sum_i sin^2(k_i dx/2) e^{ik.x} for the periodic 7-point stencil."""
+    dx = 0.37
+    f = plane_wave(N, n, 1.3, 0.4)
+    lam = -(4 / dx ** 2) * sum(math.sin(math.pi * ni / N) ** 2 for ni in n)
+    out = oracle_mod.laplacian(f, dx)
+    np.testing.assert_allclose(out, lam * f, rtol=0, atol=1e-12 * abs(lam))
+
+
+def test_lphi4_potential_matches_model_formulas(oracle_mod):
+    """dV/dphi from the polynomial equals the model header's potDeriv (models/lphi4.h:195-206) and a
+    central finite difference of V (models/lphi4.h:154-177)."""
+    q = 100.0
+    spec = lphi4(q)
+    lat = oracle_mod.OracleLattice(spec, 4, 1.0, 0.01)
+    import ctypes as C
+
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        phi, chi = rng.uniform(-2, 2, size=2)
+        buf = np.array([phi, chi, 0.0, 0.0])
+        p = buf.ctypes.data_as(C.POINTER(C.c_double))
+        m = C.byref(lat.m)
+        v = lat.lib.orc_scalar_potential(m, p)
+        d0 = lat.lib.orc_scalar_potential_deriv(m, p, 0)
+        d1 = lat.lib.orc_scalar_potential_deriv(m, p, 1)
+        assert v == pytest.approx(0.25 * phi ** 4 + 0.5 * q * (phi * chi) ** 2, rel=1e-14)
+        assert d0 == pytest.approx(phi ** 3 + q * phi * chi ** 2, rel=1e-13, abs=1e-13)
+        assert d1 == pytest.approx(q * chi * phi ** 2, rel=1e-13, abs=1e-13)
+        h = 1e-5
+        for k, d in ((0, d0), (1, d1)):
+            e = np.zeros(4)
+            e[k] = h
+            vp = lat.lib.orc_scalar_potential(m, (buf + e).ctypes.data_as(C.POINTER(C.c_double)))
+            vm = lat.lib.orc_scalar_potential(m, (buf - e).ctypes.data_as(C.POINTER(C.c_double)))
+            assert (vp - vm) / (2 * h) == pytest.approx(d, rel=1e-6, abs=1e-6)
+    # masses from potDeriv2 at the homogeneous IC (models/lphi4.h:215-226)
+    assert spec.mass2([1.0, 0.0]) == [3.0, q]
+
+
+# ------------------------------------------------------------------------------------------------
+# Evolvers: exact discrete solutions, reversibility, conservation
+# ------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("evolver", ["LF", "VV2"])
+def test_free_plane_wave_exact_recurrence(oracle_mod, evolver):
+    """Free massive field, static background: a plane wave obeys the exact leapfrog recurrence
+    phi_n = A cos(n W dt) with cos(W dt) = 1 - w^2 dt^2 / 2 and w^2 = m^2 + (4/dx^2) sum sin^2.
+    Synced momenta are pi_n = -A sin(n W dt) sin(W dt) / dt (same for VV2)."""
+    N, dx, dt, m2 = 8, 0.5, 0.05, 0.7
+    n = (1, 2, 0)
+    A = 0.3
+    lat = oracle_mod.OracleLattice(free_field(m2), N, dx, dt, expansion=False)
+    f0 = plane_wave(N, n, A)
+    lat.upload(FLD_S, 0, 0, f0)
+    w2 = m2 + (4 / dx ** 2) * sum(math.sin(math.pi * ni / N) ** 2 for ni in n)
+    W = math.acos(1 - w2 * dt * dt / 2) / dt
+    steps = 37
+    if evolver == "LF":
+        lat.lf_init()
+        lat.lf_evolve(steps)
+        lat.lf_sync()
+    else:
+        lat.vv_evolve([1.0], steps)
+    phi = lat.download(FLD_S, 0)
+    pi = lat.download(PI_S, 0)
+    shape = f0 / A
+    np.testing.assert_allclose(phi, A * math.cos(steps * W * dt) * shape, atol=1e-13)
+    np.testing.assert_allclose(pi, -A * math.sin(steps * W * dt) * math.sin(W * dt) / dt * shape, atol=1e-12)
+
+
+def test_leapfrog_time_reversible(oracle_mod):
+    """Synced leapfrog with dt -> -dt retraces its path (static background)."""
+    N, dx, dt = 8, 0.6, 0.02
+    spec = lphi4(100.0)
+    rng = np.random.default_rng(3)
+    phi0 = 1.0 + smooth_random(N, rng)
+    chi0 = smooth_random(N, rng, amp=0.05)
+    pi0 = smooth_random(N, rng, amp=0.1) - 0.5
+    fw = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    fw.upload(FLD_S, 0, 0, phi0)
+    fw.upload(FLD_S, 1, 0, chi0)
+    fw.upload(PI_S, 0, 0, pi0)
+    fw.lf_init()
+    fw.lf_evolve(25)
+    fw.lf_sync()
+    bw = oracle_mod.OracleLattice(spec, N, dx, -dt, expansion=False)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        bw.upload(*k, 0, fw.download(*k))
+    bw.lf_init()
+    bw.lf_evolve(25)
+    bw.lf_sync()
+    np.testing.assert_allclose(bw.download(FLD_S, 0), phi0, atol=1e-12)
+    np.testing.assert_allclose(bw.download(FLD_S, 1), chi0, atol=1e-12)
+    np.testing.assert_allclose(bw.download(PI_S, 0), pi0, atol=1e-12)
+    assert np.max(np.abs(fw.download(FLD_S, 0) - phi0)) > 1e-3  # it did move
+
+
+def _energy_scalar(lat):
+    av = lat.averages()
+    return 0.5 * av["pi2S"] + 0.5 * av["grad2S"] + av["pot"]
+
+
+def _max_energy_drift(oracle_mod, dt, T=1.0):
+    N, dx = 8, 0.7
+    spec = lphi4(30.0)
+    rng = np.random.default_rng(5)
+    lat = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    lat.upload(FLD_S, 0, 0, 1.0 + smooth_random(N, rng))
+    lat.upload(FLD_S, 1, 0, smooth_random(N, rng, amp=0.1))
+    lat.upload(PI_S, 0, 0, smooth_random(N, rng, amp=0.2))
+    lat.lf_init()
+    e0 = _energy_scalar(lat)
+    steps_per = int(round(0.1 / dt))
+    worst = 0.0
+    for _ in range(int(round(T / 0.1))):
+        lat.lf_evolve(steps_per)
+        lat.lf_sync()
+        worst = max(worst, abs(_energy_scalar(lat) / e0 - 1))
+    return worst
+
+
+def test_energy_conserved_second_order(oracle_mod):
+    """Without expansion the discrete energy <pi^2/2 + (grad+ phi)^2/2 + V> is conserved to O(dt^2):
+    halving dt shrinks the worst violation by ~4.  This ties the forward-difference gradient
+    energy to the 7-point kick, and V to dV/dphi."""
+    e1 = _max_energy_drift(oracle_mod, 0.02)
+    e2 = _max_energy_drift(oracle_mod, 0.01)
+    assert e1 < 5e-3
+    assert 3.5 < e1 / e2 < 4.5
+
+
+def reference_run(lat, steps, out_every, evolver="LF", ws=(1.0,)):
+    """The reference's main loop (source/cosmolattice.cpp:168-201): measure (sync + Friedmann check,
+    which runs setAllAverages) every out_every steps, then evolve.  Returns the Friedmann records."""
+    recs = []
+    if evolver == "LF":
+        lat.lf_init()
+    for i in range(steps):
+        if i % out_every == 0:
+            if evolver == "LF":
+                lat.lf_sync()
+            recs.append(lat.hubble_constraint())
+        if evolver == "LF":
+            lat.lf_evolve(1)
+        else:
+            lat.vv_evolve(list(ws), 1)
+    if evolver == "LF":
+        lat.lf_sync()
+    recs.append(lat.hubble_constraint())
+    return recs
+
+
+def _init_expanding(oracle_mod, spec, N, dx, dt, seed=7):
+    rng = np.random.default_rng(seed)
+    lat = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=True)
+    lat.upload(FLD_S, 0, 0, spec.fld0[0] + smooth_random(N, rng, amp=0.05))
+    if spec.n_scalars > 1:
+        lat.upload(FLD_S, 1, 0, smooth_random(N, rng, amp=0.05))
+    lat.upload(PI_S, 0, 0, spec.pi0[0] + smooth_random(N, rng, amp=0.05))
+    # ModelInitializer::initialize (initializers/modelinitializer.h:71-80): aDotI = sqrt(RHS)
+    lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=1.0)
+    _, _, rhs = lat.hubble_constraint()
+    lat.set_background(aDotI=math.sqrt(rhs))
+    return lat
+
+
+@pytest.mark.parametrize("evolver", ["LF", "VV2"])
+def test_friedmann_constraint_second_order(oracle_mod, evolver):
+    """With self-consistent expansion the first Friedmann equation holds to O(dt^2): the worst
+    |rel_diff_friedmann| shrinks ~4x when dt halves.  A wrong sign or factor in the scale-factor
+    kernel (ScaleFactorKernels::get), the energies, or the leapfrog's a / aDot staggering breaks
+    this."""
+    spec = lphi4(50.0)
+    worst = []
+    for dt in (0.02, 0.01):
+        lat = _init_expanding(oracle_mod, spec, 8, 0.8, dt)
+        recs = reference_run(lat, int(round(1.0 / dt)), int(round(0.1 / dt)), evolver)
+        worst.append(max(abs(r[0]) for r in recs))
+        assert lat.background()["aI"] > 1.2  # expansion is not inert
+    assert worst[0] < 1e-3
+    assert 3.0 < worst[0] / worst[1] < 5.2
+
+
+def test_setallaverages_overwrites_half_step_cache(oracle_mod):
+    """Averages::setAllAverages (definitions/averages.h:86-94) replaces pi2AvSI by the integer-time
+    average; the next leapfrog step then averages that value into pi2AvI."""
+    spec = lphi4(10.0)
+    lat = _init_expanding(oracle_mod, spec, 6, 0.8, 0.02)
+    lat.lf_init()
+    lat.lf_evolve(3)
+    lat.lf_sync()
+    before = lat.background()["pi2AvSI"]
+    lat.set_all_averages()
+    after = lat.background()
+    assert after["pi2AvSI"] == after["pi2AvI"] != before
+    assert after["potAvSI"] == after["potAvI"]
+
+
+# ------------------------------------------------------------------------------------------------
+# U(1) sector
+# ------------------------------------------------------------------------------------------------
+
+def _u1_state(N, rng, amp=0.2):
+    f = {}
+    f[(FLD_CS, 0)] = 0.7 + smooth_random(N, rng, amp=amp)
+    f[(FLD_CS, 1)] = 0.7 + smooth_random(N, rng, amp=amp)
+    f[(PI_CS, 0)] = smooth_random(N, rng, amp=amp)
+    f[(PI_CS, 1)] = smooth_random(N, rng, amp=amp)
+    for d in range(3):
+        f[(FLD_U1, d)] = smooth_random(N, rng, amp=amp)
+        f[(PI_U1, d)] = smooth_random(N, rng, amp=amp)
+    return f
+
+
+def _load(lat, state):
+    for (k, c), a in state.items():
+        lat.upload(k, 0, c, a)
+
+
+def _u1_spec(gQ=1.3):
+    spec = lphi4U1()
+    spec.fStar, spec.omegaStar = 1.0, 1.0
+    spec.gQ = gQ
+    return spec
+
+
+def test_u1_gauge_invariance(oracle_mod):
+    """phi -> e^{i chi} phi, A_i -> A_i + (chi(x+i) - chi(x))/(dx gQ) with links exp(-i dx gQ A):
+    energies are invariant and one leapfrog step is covariant."""
+    N, dx, dt = 6, 0.9, 0.03
+    spec = _u1_spec()
+    rng = np.random.default_rng(11)
+    st = _u1_state(N, rng)
+    chi = rng.uniform(-3, 3, size=(N, N, N))
+    g = {}
+    ph = np.exp(1j * chi)
+    for (kf, kp) in ((FLD_CS, FLD_CS), (PI_CS, PI_CS)):
+        z = (st[(kf, 0)] + 1j * st[(kf, 1)]) * ph
+        g[(kf, 0)], g[(kf, 1)] = z.real, z.imag
+    for d in range(3):
+        g[(FLD_U1, d)] = st[(FLD_U1, d)] + (np.roll(chi, -1, axis=d) - chi) / (dx * spec.gQ)
+        g[(PI_U1, d)] = st[(PI_U1, d)]
+    a = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    b = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    _load(a, st)
+    _load(b, g)
+    av, bv = a.averages(), b.averages()
+    for k in ("pot", "pi2CS", "grad2CS", "pi2U1", "B2U1"):
+        assert bv[k] == pytest.approx(av[k], rel=1e-11)
+    a.lf_init(), b.lf_init()
+    a.lf_evolve(1), b.lf_evolve(1)
+    a.lf_sync(), b.lf_sync()
+    za = a.download(PI_CS, 0, 0) + 1j * a.download(PI_CS, 0, 1)
+    zb = b.download(PI_CS, 0, 0) + 1j * b.download(PI_CS, 0, 1)
+    # the step moved A, so the gauge function is unchanged only up to the drift of A, which is
+    # gauge invariant (pi_A is); phi picks up the same phase
+    np.testing.assert_allclose(zb, za * ph, atol=1e-11)
+    for d in range(3):
+        np.testing.assert_allclose(b.download(PI_U1, 0, d), a.download(PI_U1, 0, d), atol=1e-11)
+
+
+def test_u1_gauss_law_conserved(oracle_mod):
+    """Initial data obeying the discrete Gauss law (pi_A = 0, charge density zero) keep obeying it
+    under leapfrog evolution: the kick's current and the charge density satisfy the discrete
+    continuity equation."""
+    N, dx, dt = 8, 0.8, 0.02
+    spec = _u1_spec(gQ=2.0)
+    rng = np.random.default_rng(13)
+    st = _u1_state(N, rng)
+    s = smooth_random(N, rng, amp=0.3)
+    st[(PI_CS, 0)] = s * st[(FLD_CS, 0)]
+    st[(PI_CS, 1)] = s * st[(FLD_CS, 1)]
+    for d in range(3):
+        st[(PI_U1, d)] = np.zeros((N, N, N))
+    lat = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    _load(lat, st)
+    lat.lf_init()
+    g0 = lat.gauss_u1()
+    assert g0[1] < 1e-14 and g0[2] < 1e-14
+    lat.lf_evolve(40)
+    lat.lf_sync()
+    diff, lhs, rhs = lat.gauss_u1()
+    assert lhs > 1e-3  # electric field and charges did develop
+    assert diff < 1e-12
+
+
+def _u1_energy(lat):
+    av = lat.averages()
+    w2 = (lat.spec.omegaStar / lat.spec.fStar) ** 2
+    return av["pi2CS"] + av["grad2CS"] + av["pot"] + 0.5 * w2 * (av["pi2U1"] + av["B2U1"])
+
+
+def _u1_energy_drift(oracle_mod, dt):
+    N, dx = 6, 0.9
+    spec = _u1_spec(gQ=1.1)
+    spec.omegaStar = 0.8  # exercises the (omegaStar/fStar)^2 factors of the gauge energies
+    rng = np.random.default_rng(17)
+    lat = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    _load(lat, _u1_state(N, rng))
+    lat.lf_init()
+    e0 = _u1_energy(lat)
+    worst = 0.0
+    per = int(round(0.1 / dt))
+    for _ in range(6):
+        lat.lf_evolve(per)
+        lat.lf_sync()
+        worst = max(worst, abs(_u1_energy(lat) / e0 - 1))
+    return worst
+
+
+def test_u1_energy_conserved_second_order(oracle_mod):
+    """Charged scalar + U(1) without expansion: the energy with the reference's normalisations
+    (definitions/energies.h:47-85) is conserved to O(dt^2)."""
+    e1 = _u1_energy_drift(oracle_mod, 0.02)
+    e2 = _u1_energy_drift(oracle_mod, 0.01)
+    assert e1 < 2e-3
+    assert 3.0 < e1 / e2 < 5.0
+
+
+def test_charged_scalar_without_gauge_coupling_is_two_real_scalars(oracle_mod):
+    """gQ = 0 and V = |phi|^4: the complex scalar evolves exactly like two real scalars with
+    V = (phi1^2 + phi2^2)^2 / 2 (kick factor aV/2 of complexscalarkernels.h:33-34)."""
+    N, dx, dt = 6, 0.8, 0.02
+    spec_c = _u1_spec(gQ=0.0)
+    spec_s = ModelSpec(name="two", n_scalars=2, terms=[(0.5, [4, 0]), (1.0, [2, 2]), (0.5, [0, 4])])
+    rng = np.random.default_rng(19)
+    st = _u1_state(N, rng)
+    c = oracle_mod.OracleLattice(spec_c, N, dx, dt, expansion=False)
+    s = oracle_mod.OracleLattice(spec_s, N, dx, dt, expansion=False)
+    _load(c, st)
+    for k in range(2):
+        s.upload(FLD_S, k, 0, st[(FLD_CS, k)])
+        s.upload(PI_S, k, 0, st[(PI_CS, k)])
+    c.lf_init(), s.lf_init()
+    c.lf_evolve(15), s.lf_evolve(15)
+    c.lf_sync(), s.lf_sync()
+    for k in range(2):
+        np.testing.assert_allclose(c.download(FLD_CS, 0, k), s.download(FLD_S, k), rtol=0, atol=1e-13)
+        np.testing.assert_allclose(c.download(PI_CS, 0, k), s.download(PI_S, k), rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------------
+# Spectra and Gaussian random fields
+# ------------------------------------------------------------------------------------------------
+
+def test_spectrum_parseval_and_counts(oracle_mod):
+    """Sum over bins >= 1 of sum|f_n|^2 / N^6 equals the variance of f (Parseval), and bin counts
+    match a brute-force enumeration of integer mode vectors."""
+    N = 12
+    rng = np.random.default_rng(23)
+    f = rng.normal(size=(N, N, N)) + 0.3
+    s, cnt, mean = oracle_mod.spectrum_bins(f, 1.0)
+    assert s[1:].sum() / N ** 6 == pytest.approx(f.var(), rel=1e-12)
+    assert cnt[0] == 1 and s[0] == pytest.approx((f.sum()) ** 2, rel=1e-12)
+    brute = np.zeros_like(cnt)
+    rng_n = [i if i <= N // 2 else i - N for i in range(N)]
+    for a in rng_n:
+        for b in rng_n:
+            for c in rng_n:
+                r = math.sqrt(a * a + b * b + c * c)
+                brute[min(int(math.floor(r + 0.5)), len(cnt) - 1)] += 1
+    np.testing.assert_array_equal(cnt, brute)
+    assert cnt.sum() == N ** 3
+
+
+def test_spectrum_plane_wave_single_bin(oracle_mod):
+    """A plane wave of amplitude A puts A^2 N^6 / 2 into the bin of |n| and nothing elsewhere."""
+    N = 16
+    n = (3, 4, 0)  # |n| = 5
+    A = 0.7
+    s, _, mean = oracle_mod.spectrum_bins(plane_wave(N, n, A, 0.3), 1.0)
+    assert s[5] == pytest.approx(A * A * N ** 6 / 2, rel=1e-12)
+    assert np.sum(np.delete(s, 5)) < 1e-12 * s[5]
+
+
+def test_grf_statistics_and_structure(oracle_mod):
+    """Vacuum modes: E|phi_k|^2 = (w*/f*)^2 (L/dx^2)^3 / (2 w_k) below the cut, zero above and at
+    k = 0; the real-space field reproduces the half-space modes (exact Hermitian completion);
+    pi_k + aDot phi_k is uncorrelated with phi_k and has power w_k^2 E|phi_k|^2."""
+    N, kIR, m2, aDot = 16, 0.3, 1.7, 0.4
+    dx = 2 * math.pi / kIR / N
+    L = N * dx
+    spec = lphi4(1.0)
+    spec.fStar, spec.omegaStar = 1.0, 1.0
+    kcut = 6.5 * kIR
+    pref = (L / dx ** 2) ** 1.5 / math.sqrt(2)
+    phik, pik = oracle_mod.grf_modes(N, 42, 3, m2, aDot, kIR, kcut, pref)
+    Nh = N // 2 + 1
+    kx, ky, kz = np.meshgrid(np.arange(N), np.arange(N), np.arange(Nh), indexing="ij")
+    nx = np.where(kx <= N // 2, kx, kx - N)
+    ny = np.where(ky <= N // 2, ky, ky - N)
+    n2 = nx ** 2 + ny ** 2 + kz ** 2
+    k = np.sqrt(n2) * kIR
+    keep = (n2 > 0) & (k <= kcut)
+    assert np.all(phik[~keep] == 0) and np.all(pik[~keep] == 0)
+    om = np.sqrt(k ** 2 + m2)
+    expect = (L / dx ** 2) ** 3 / (2 * om)
+    ratio = (np.abs(phik) ** 2 / expect)[keep & (kz > 0) & (2 * kz < N)]
+    assert abs(ratio.mean() - 1) < 5 / math.sqrt(ratio.size)
+    d = pik + aDot * phik
+    r2 = (np.abs(d) ** 2 / (om ** 2 * expect))[keep & (kz > 0) & (2 * kz < N)]
+    assert abs(r2.mean() - 1) < 5 / math.sqrt(r2.size)
+    cross = (np.real(np.conj(phik) * d) / (om * expect))[keep & (kz > 0) & (2 * kz < N)]
+    assert abs(cross.mean()) < 5 / math.sqrt(cross.size)
+    phi, pi = oracle_mod.grf_scalar(N, dx, spec, 42, 3, m2, aDot, kIR, kcut, 0.25, -0.5)
+    back = np.fft.rfftn(phi - 0.25)
+    np.testing.assert_allclose(back, phik, atol=1e-9 * np.abs(phik).max())
+    assert phi.mean() == pytest.approx(0.25, abs=1e-12)
+    assert pi.mean() == pytest.approx(-0.5, abs=1e-12)
