@@ -110,7 +110,7 @@ def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
         ref.lf_evolve(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 50:
+        if el >= budget_s or steps >= 2000:
             break
     return {
         "value": N ** 3 * steps / el,
@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (torch) so HBM counters can be scaled")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -179,6 +181,11 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "kernel": "k_scalar_pass (fused drift+kick+reductions)",
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"]}
+
+    if args.calibrate_copy:
+        # A plain copy kernel of known size (k_copy): its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the counter units against 1 GiB read + 1 GiB written per launch.
+        lats["f64"].copy_calibration(1 << 30, 3)
 
     cpu = None
     if not args.no_cpu:

@@ -22,6 +22,7 @@ OBJDIR = PKG / "_objs"
 
 HIP_SOURCES = [
     "scalar_pass.hip",
+    "scalar_tiled.hip",
     "gauge_pass.hip",
     "background.hip",
     "fft_kernels.hip",

@@ -29,6 +29,7 @@ CLMI_F64 = 0
 CLMI_F32_BULK = 1
 CLMI_MAX_SCALARS = 4
 CLMI_MAX_TERMS = 8
+CLMI_FLAG_NO_TILED = 1
 
 FLD_S, PI_S, FLD_CS, PI_CS, FLD_U1, PI_U1 = 0, 1, 2, 3, 6, 7
 
@@ -40,7 +41,7 @@ REDUCED_MPL = 2.435e18
 class ClmiConfig(C.Structure):
     _fields_ = [
         ("N", C.c_int), ("n_scalars", C.c_int), ("n_cscalars", C.c_int), ("n_u1", C.c_int),
-        ("precision", C.c_int), ("expansion", C.c_int), ("device", C.c_int), ("reserved", C.c_int),
+        ("precision", C.c_int), ("expansion", C.c_int), ("device", C.c_int), ("flags", C.c_int),
         ("dx", C.c_double), ("dt", C.c_double),
         ("alpha", C.c_double), ("fStar", C.c_double), ("omegaStar", C.c_double), ("MPl", C.c_double),
         ("gQ", C.c_double),
@@ -114,6 +115,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
                                       C.c_double, C.c_double, C.c_double]),
         "clmi_profile_enable": (C.c_int, [P, C.c_int]),
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -211,8 +213,9 @@ def lphi4U1(e2_over_lambda: float = 10.0, phi0_gev: float = 5.6964e18, lam: floa
 
 
 def make_config(spec: ModelSpec, N: int, dx: float, dt: float, precision: int = CLMI_F64,
-                expansion: bool = True, device: int = 0) -> ClmiConfig:
+                expansion: bool = True, device: int = 0, flags: int = 0) -> ClmiConfig:
     cfg = ClmiConfig()
+    cfg.flags = flags
     cfg.N = N
     cfg.n_scalars = spec.n_scalars
     cfg.n_cscalars = spec.n_cscalars
@@ -243,13 +246,13 @@ class Lattice:
     """One device context: the fields, momenta and background of a model on an N^3 lattice."""
 
     def __init__(self, spec: ModelSpec, N: int, dx: float, dt: float, precision: int = CLMI_F64,
-                 expansion: bool = True, device: int = 0):
+                 expansion: bool = True, device: int = 0, flags: int = 0):
         self.lib = load_library()
         self.spec = spec
         self.N, self.dx, self.dt = N, dx, dt
         self.precision = precision
         self.expansion = expansion
-        self.cfg = make_config(spec, N, dx, dt, precision, expansion, device)
+        self.cfg = make_config(spec, N, dx, dt, precision, expansion, device, flags)
         h = C.c_void_p()
         rc = self.lib.clmi_create(C.byref(self.cfg), C.byref(h))
         if rc != CLMI_OK:
@@ -387,6 +390,9 @@ class Lattice:
         b = C.c_double()
         self._check(self.lib.clmi_profile_read(self._h, C.byref(ms), C.byref(n), C.byref(b)), "profile_read")
         return ms.value, n.value, b.value
+
+    def copy_calibration(self, nbytes: int, iters: int = 3):
+        self._check(self.lib.clmi_copy_calibration(self._h, nbytes, iters), "copy_calibration")
 
 
 def power_spectrum(sum_f2, count, mean_n, N: int, kIR: float, dk: float = 1.0, ps_type: int = 1,

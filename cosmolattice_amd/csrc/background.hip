@@ -257,6 +257,12 @@ __global__ void k_join(const float *hi, const float *lo, double *dst, uint32_t V
     dst[i] = (double)hi[i] + (lo ? (double)lo[i] : 0.0);
 }
 
+// Plain streaming copy (16-byte vectors), used only to calibrate hardware byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *src, float4 *dst, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 int conv_blocks(uint32_t V)
 {
   uint32_t b = (V + 255) / 256;
@@ -293,6 +299,13 @@ hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s)
 hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s)
 {
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, s, partials, rows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+  const size_t n = bytes / sizeof(float4);
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
   return hipGetLastError();
 }
 

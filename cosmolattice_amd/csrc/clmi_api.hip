@@ -43,6 +43,7 @@ struct clmi_ctx {
   double *partials = nullptr;
   double *dscratch = nullptr;  // small device scratch for row sums
   int blocks_s = 0, blocks_g = 0;
+  int tiled_xc = 0;  // > 0: scalar passes use the tiled kernel with this x-chunk
 
   ScalarPoly spoly{};
   CsPoly cpoly{};
@@ -191,7 +192,11 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     a.st = c->dstate;
     a.partials = c->partials + (size_t)r * kNumRed;
     a.poly = c->spoly;
-    HIPCHK(c, launch_scalar_pass(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->stream));
+    if (c->tiled_xc > 0)
+      HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->tiled_xc,
+                                    c->stream));
+    else
+      HIPCHK(c, launch_scalar_pass(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->stream));
     if (red) r += c->blocks_s;
     c->cur_fS = fo;
     c->cur_pS = po;
@@ -401,7 +406,15 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
 
   c->blocks_s = scalar_pass_blocks(c->V);
   c->blocks_g = gauge_pass_blocks(c->V);
-  const size_t rows = (size_t)(c->blocks_s + c->blocks_g) + 16;
+  if (!(g.flags & CLMI_FLAG_NO_TILED)) {
+    int tb = 0, xc = 0;
+    scalar_tiled_geometry(g.N, &tb, &xc);
+    if (tb > 0 && tb <= kMaxPassBlocks) {
+      c->blocks_s = tb;
+      c->tiled_xc = xc;
+    }
+  }
+  const size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
   if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dstate, sizeof(DevState)) != hipSuccess) {
@@ -865,6 +878,26 @@ int clmi_profile_enable(clmi_ctx *c, int on)
   c->prof = on != 0;
   c->prof_ms = 0.0;
   c->prof_launches = 0;
+  return CLMI_OK;
+}
+
+int clmi_copy_calibration(clmi_ctx *c, size_t bytes, int iters)
+{
+  CHK(check_ctx(c));
+  if (bytes < 16 || bytes > ((size_t)16 << 30) || iters < 1) return fail(c, CLMI_E_INVALID, "bad copy size");
+  bytes &= ~(size_t)15;
+  void *s = nullptr, *d = nullptr;
+  HIPCHK(c, hipMalloc(&s, bytes));
+  if (hipMalloc(&d, bytes) != hipSuccess) {
+    (void)hipFree(s);
+    return fail(c, CLMI_E_NOMEM, "copy buffer allocation failed");
+  }
+  hipError_t e = hipMemsetAsync(s, 0, bytes, c->stream);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_copy(s, d, bytes, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(s);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, CLMI_E_HIP, std::string("copy calibration: ") + hipGetErrorString(e));
   return CLMI_OK;
 }
 

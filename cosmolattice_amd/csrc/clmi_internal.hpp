@@ -165,6 +165,10 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
 }
 
 // Launchers (defined in the .hip files).
+void scalar_tiled_geometry(int N, int *blocks, int *XC);
+hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
+                               int blocks, int XC, hipStream_t s);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 int scalar_pass_blocks(uint32_t V);
 hipError_t launch_scalar_pass(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               int blocks, hipStream_t s);

@@ -1,0 +1,401 @@
+// Tiled scalar-singlet lattice pass for N a multiple of 64: same arithmetic as scalar_pass.hip
+// (bit-identical results), organised for HBM streaming on CDNA4.
+//
+// Each workgroup (4 wavefronts) owns a 64 (z) x TY (y) column of sites and marches along x over
+// XC planes.  Along x the stencil neighbours live in registers (prev / cur / next plane); within a
+// plane the y and z neighbours come from a double-buffered LDS tile with a one-site halo.  The halo
+// sites are loaded and drifted by dedicated lanes, so every HBM array is read once per pass plus
+// the halo (2 rows + 2 columns per plane, and 2 extra planes per x-chunk).  Loads for plane x+2 are
+// issued before plane x is computed to hide HBM latency.  One barrier per plane.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; the block index is remapped so that the
+// workgroups of one XCD own neighbouring tiles and x-chunks, which keeps halo re-reads in that
+// XCD's L2.
+#include "clmi_internal.hpp"
+
+namespace clmi {
+namespace {
+
+constexpr int TZ = 64;
+
+__device__ __forceinline__ double ipow_d(double v, int e)
+{
+  double r = 1.0;
+  for (int i = 0; i < e; ++i) r *= v;
+  return r;
+}
+__device__ __forceinline__ float ipow_f(float v, int e)
+{
+  float r = 1.0f;
+  for (int i = 0; i < e; ++i) r *= v;
+  return r;
+}
+template <int NF>
+__device__ __forceinline__ double dpot_d(const ScalarPoly &P, const double *phi, int n)
+{
+  double d = 0.0;
+  for (int t = 0; t < P.nterms; ++t) {
+    const int e = P.e[t][n];
+    if (e == 0) continue;
+    double p = P.c[t] * e;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], f == n ? e - 1 : P.e[t][f]);
+    d += p;
+  }
+  return d;
+}
+template <int NF>
+__device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, int n)
+{
+  float d = 0.0f;
+  for (int t = 0; t < P.nterms; ++t) {
+    const int e = P.e[t][n];
+    if (e == 0) continue;
+    float p = (float)P.c[t] * (float)e;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], f == n ? e - 1 : P.e[t][f]);
+    d += p;
+  }
+  return d;
+}
+template <int NF>
+__device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
+{
+  double v = 0.0;
+  for (int t = 0; t < P.nterms; ++t) {
+    double p = P.c[t];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], P.e[t][f]);
+    v += p;
+  }
+  return v;
+}
+__device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
+{
+  const float s = lo + d;
+  const float h = hi + s;
+  const float bb = h - hi;
+  const float e = (hi - (h - bb)) + (s - bb);
+  hi = h;
+  lo = e;
+}
+
+// Storage traits: a drifted field value (FP64: double; FP32 bulk: hi/lo pair) and a momentum.
+template <bool F32>
+struct Tr;
+template <>
+struct Tr<false> {
+  using V = double;
+  using P = double;
+  struct Raw {
+    double f;
+    double p;
+  };
+  __device__ static __forceinline__ Raw load(const ScalarPassArgs &a, int f, uint32_t i, bool need_p)
+  {
+    Raw r;
+    r.f = ((const double *)a.phi_in[f])[i];
+    r.p = need_p ? ((const double *)a.pi_in[f])[i] : 0.0;
+    return r;
+  }
+  template <bool DRIFT>
+  __device__ static __forceinline__ V value(const Raw &r, double fs, double cdt, float)
+  {
+    double v = r.f;
+    if (DRIFT) v = __dadd_rn(v, __dmul_rn(fs, __dmul_rn(cdt, r.p)));
+    return v;
+  }
+};
+template <>
+struct Tr<true> {
+  using V = float2;
+  using P = float;
+  struct Raw {
+    float h, l, p;
+  };
+  __device__ static __forceinline__ Raw load(const ScalarPassArgs &a, int f, uint32_t i, bool need_p)
+  {
+    Raw r;
+    r.h = ((const float *)a.phi_in[f])[i];
+    r.l = a.philo_in[f][i];
+    r.p = need_p ? ((const float *)a.pi_in[f])[i] : 0.0f;
+    return r;
+  }
+  template <bool DRIFT>
+  __device__ static __forceinline__ V value(const Raw &r, double, double, float cf)
+  {
+    float h = r.h, l = r.l;
+    if (DRIFT) pair_add(h, l, cf * r.p);
+    return make_float2(h, l);
+  }
+};
+
+template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY>
+__global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a, const int XC)
+{
+  using T = Tr<F32>;
+  using V = typename T::V;
+  using Raw = typename T::Raw;
+  constexpr int TY = 4 * RY;
+  __shared__ V lds[2][NF][TY + 2][TZ + 2];
+
+  const int N = a.N;
+  const uint32_t NN = (uint32_t)N * N;
+  const int nTz = N / TZ, nTy = N / TY;
+  const int nwg = gridDim.x;
+  const int b = blockIdx.x;
+  const int L = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
+  const int tz = L % nTz, ty = (L / nTz) % nTy, cx = L / (nTz * nTy);
+  const int z0 = tz * TZ, y0 = ty * TY, x0 = cx * XC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int z = z0 + lane;
+
+  const double fs = DRIFT ? a.st->fdrift_s : 0.0;
+  const double aK = KICK ? a.st->aK : 0.0;
+  const double aV = KICK ? a.st->aV : 0.0;
+  const float cf = DRIFT ? (float)(a.st->fdrift_s * a.cdt) : 0.0f;
+  const float kK = KICK ? (float)(a.ckick * a.st->aK * a.inv_dx2) : 0.0f;
+  const float kV = KICK ? (float)(a.ckick * a.st->aV) : 0.0f;
+  const bool need_p_nb = DRIFT;  // momenta of neighbouring planes are read only to drift them
+
+  auto site = [&](int x, int y, int zz) -> uint32_t {
+    x = x < 0 ? x + N : (x >= N ? x - N : x);
+    y = y < 0 ? y + N : (y >= N ? y - N : y);
+    zz = zz < 0 ? zz + N : (zz >= N ? zz - N : zz);
+    return (uint32_t)x * NN + (uint32_t)y * N + zz;
+  };
+
+  // halo lanes: wave 0 -> row y0-1, wave 1 -> row y0+TY, waves 2/3 lanes < TY -> columns z0-1 / z0+TZ
+  int hy = 0, hz = 0, hly = 0, hlz = 0;
+  bool has_halo = false;
+  if (wave == 0) { hy = y0 - 1; hz = z; hly = 0; hlz = lane + 1; has_halo = true; }
+  else if (wave == 1) { hy = y0 + TY; hz = z; hly = TY + 1; hlz = lane + 1; has_halo = true; }
+  else if (lane < TY) {
+    hy = y0 + lane;
+    hly = lane + 1;
+    if (wave == 2) { hz = z0 - 1; hlz = 0; } else { hz = z0 + TZ; hlz = TZ + 1; }
+    has_halo = true;
+  }
+
+  V prev[RY][NF], cur[RY][NF], nxt[RY][NF];
+  typename T::P pic[RY][NF], pin[RY][NF];
+  Raw pf[RY][NF];
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+
+  // prologue: planes x0-1, x0, x0+1 and the LDS tile of plane x0
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + wave * RY + r;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      Raw q = T::load(a, f, site(x0 - 1, y, z), need_p_nb);
+      prev[r][f] = T::template value<DRIFT>(q, fs, a.cdt, cf);
+      q = T::load(a, f, site(x0, y, z), true);
+      cur[r][f] = T::template value<DRIFT>(q, fs, a.cdt, cf);
+      pic[r][f] = q.p;
+      q = T::load(a, f, site(x0 + 1, y, z), true);
+      nxt[r][f] = T::template value<DRIFT>(q, fs, a.cdt, cf);
+      pin[r][f] = q.p;
+      lds[0][f][wave * RY + r + 1][lane + 1] = cur[r][f];
+    }
+  }
+  if (has_halo) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const Raw q = T::load(a, f, site(x0, hy, hz), need_p_nb);
+      lds[0][f][hly][hlz] = T::template value<DRIFT>(q, fs, a.cdt, cf);
+    }
+  }
+  __syncthreads();
+
+  for (int i = 0; i < XC; ++i) {
+    const int x = x0 + i;
+    const int buf = i & 1;
+    // prefetch plane x+2 (own sites) and the halo of plane x+1
+    const bool more = i + 1 < XC;
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) pf[r][f] = T::load(a, f, site(x + 2, y0 + wave * RY + r, z), true);
+    }
+    Raw hq[NF];
+    if (more && has_halo) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) hq[f] = T::load(a, f, site(x + 1, hy, hz), need_p_nb);
+    }
+
+    // compute plane x
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int ly = wave * RY + r + 1;
+      const uint32_t s = site(x, y0 + ly - 1, z);
+      double g2 = 0.0;
+      double phid[NF];
+      float phif[NF];
+      double lapd[NF];
+      float lapf[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const V c = cur[r][f];
+        const V vyp = lds[buf][f][ly + 1][lane + 1], vym = lds[buf][f][ly - 1][lane + 1];
+        const V vzp = lds[buf][f][ly][lane + 2], vzm = lds[buf][f][ly][lane];
+        const V vxp = nxt[r][f], vxm = prev[r][f];
+        if constexpr (!F32) {
+          phid[f] = c;
+          if (DRIFT) ((double *)a.phi_out[f])[s] = c;
+          if (KICK) {
+            double l = 0.0;
+            l += vxp + vxm - 2 * c;
+            l += vyp + vym - 2 * c;
+            l += vzp + vzm - 2 * c;
+            lapd[f] = l * a.inv_dx2;
+          }
+          if (RED) {
+            const double gx = (vxp - c) * a.inv_dx, gy = (vyp - c) * a.inv_dx, gz = (vzp - c) * a.inv_dx;
+            g2 += gx * gx;
+            g2 += gy * gy;
+            g2 += gz * gz;
+          }
+        } else {
+          phif[f] = c.x + c.y;
+          phid[f] = (double)c.x + (double)c.y;
+          if (DRIFT) {
+            ((float *)a.phi_out[f])[s] = c.x;
+            a.philo_out[f][s] = c.y;
+          }
+          const float dxp = (vxp.x - c.x) + (vxp.y - c.y);
+          const float dyp = (vyp.x - c.x) + (vyp.y - c.y);
+          const float dzp = (vzp.x - c.x) + (vzp.y - c.y);
+          if (KICK) {
+            const float dxm = (vxm.x - c.x) + (vxm.y - c.y);
+            const float dym = (vym.x - c.x) + (vym.y - c.y);
+            const float dzm = (vzm.x - c.x) + (vzm.y - c.y);
+            lapf[f] = ((dxp + dxm) + (dyp + dym)) + (dzp + dzm);
+          }
+          if (RED) {
+            const double gx = (double)dxp * a.inv_dx, gy = (double)dyp * a.inv_dx, gz = (double)dzp * a.inv_dx;
+            g2 += gx * gx;
+            g2 += gy * gy;
+            g2 += gz * gz;
+          }
+        }
+      }
+      double p2 = 0.0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if constexpr (!F32) {
+          double p = pic[r][f];
+          if (KICK) {
+            p += a.ckick * (aK * lapd[f] - aV * dpot_d<NF>(a.poly, phid, f));
+            ((double *)a.pi_out[f])[s] = p;
+          }
+          p2 += p * p;
+        } else {
+          float p = pic[r][f];
+          if (KICK) {
+            p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
+            ((float *)a.pi_out[f])[s] = p;
+          }
+          p2 += (double)p * (double)p;
+        }
+      }
+      if (RED) {
+        acc0 += p2;
+        acc1 += g2;
+        acc2 += pot_d<NF>(a.poly, phid);
+      }
+    }
+
+    if (more) {
+      // LDS tile of plane x+1
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) lds[buf ^ 1][f][wave * RY + r + 1][lane + 1] = nxt[r][f];
+      if (has_halo) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) lds[buf ^ 1][f][hly][hlz] = T::template value<DRIFT>(hq[f], fs, a.cdt, cf);
+      }
+      // rotate the register queue
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          prev[r][f] = cur[r][f];
+          cur[r][f] = nxt[r][f];
+          pic[r][f] = pin[r][f];
+          nxt[r][f] = T::template value<DRIFT>(pf[r][f], fs, a.cdt, cf);
+          pin[r][f] = pf[r][f].p;
+        }
+      __syncthreads();
+    }
+  }
+
+  if (RED) {
+    double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
+    __syncthreads();
+    block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
+  }
+}
+
+template <int NF, bool F32>
+hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC,
+                          hipStream_t s)
+{
+  constexpr int RY = 4;
+  const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
+#define CLMI_ST(D, K, R)                                                                                     \
+  case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                           \
+    hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY>), dim3(blocks), dim3(kBlock), 0, s, a, XC);     \
+    break;
+  switch (code) {
+    CLMI_ST(true, true, true)
+    CLMI_ST(true, true, false)
+    CLMI_ST(true, false, true)
+    CLMI_ST(true, false, false)
+    CLMI_ST(false, true, true)
+    CLMI_ST(false, true, false)
+    CLMI_ST(false, false, true)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CLMI_ST
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Tile geometry for N (0 blocks: the tiled pass does not apply and the generic pass is used).
+void scalar_tiled_geometry(int N, int *blocks, int *XC)
+{
+  constexpr int TY = 16;
+  *blocks = 0;
+  *XC = 0;
+  if (N % TZ != 0 || N % TY != 0) return;
+  const long long tiles = (long long)(N / TZ) * (N / TY);
+  int xc = 32;
+  while (xc > 4 && tiles * (N / xc) < 1024) xc /= 2;
+  while (N % xc != 0) xc /= 2;
+  if (xc < 2) return;
+  *XC = xc;
+  *blocks = (int)(tiles * (N / xc));
+}
+
+hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
+                               int blocks, int XC, hipStream_t s)
+{
+  switch (nf * 2 + (f32 ? 1 : 0)) {
+    case 2: return dispatch_tiled<1, false>(drift, kick, red, a, blocks, XC, s);
+    case 3: return dispatch_tiled<1, true>(drift, kick, red, a, blocks, XC, s);
+    case 4: return dispatch_tiled<2, false>(drift, kick, red, a, blocks, XC, s);
+    case 5: return dispatch_tiled<2, true>(drift, kick, red, a, blocks, XC, s);
+    case 6: return dispatch_tiled<3, false>(drift, kick, red, a, blocks, XC, s);
+    case 7: return dispatch_tiled<3, true>(drift, kick, red, a, blocks, XC, s);
+    case 8: return dispatch_tiled<4, false>(drift, kick, red, a, blocks, XC, s);
+    case 9: return dispatch_tiled<4, true>(drift, kick, red, a, blocks, XC, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace clmi

@@ -39,6 +39,10 @@ extern "C" {
 #define CLMI_F64 0
 #define CLMI_F32_BULK 1
 
+/* clmi_config.flags: use the generic one-site-per-thread lattice pass even where the tiled
+ * (x-marching, LDS) pass applies.  Both give bit-identical results; the flag exists for testing. */
+#define CLMI_FLAG_NO_TILED 1
+
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
 #define CLMI_FLD_S 0
 #define CLMI_PI_S 1
@@ -63,7 +67,7 @@ typedef struct clmi_config {
   int precision;  /* CLMI_F64 or CLMI_F32_BULK */
   int expansion;  /* 1: self-consistent expansion (runparameters.h:66); 0: static a = aI */
   int device;     /* HIP device ordinal */
-  int reserved;
+  int flags;      /* CLMI_FLAG_* bits */
   double dx, dt;
   double alpha, fStar, omegaStar, MPl; /* program units (modelparametersbase.h:20, scalefactorbase.h:18) */
   double gQ;                           /* effective U(1) charge g*Q (couplingsmanager.h:70-94) */
@@ -192,6 +196,10 @@ int clmi_profile_enable(clmi_ctx *ctx, int on);
 /* Total milliseconds and number of timed launches since enabling; bytes_per_launch is the
  * algorithmic HBM traffic of one such pass (every evolved array read once and written once). */
 int clmi_profile_read(clmi_ctx *ctx, double *total_ms, int64_t *launches, double *bytes_per_launch);
+/* No counterpart in the reference: runs `iters` launches of a plain device-to-device copy kernel of
+ * `bytes` bytes on the context stream, so that hardware byte counters (FETCH_SIZE / WRITE_SIZE)
+ * can be scaled against a kernel whose traffic is known.  Allocates and frees its two buffers. */
+int clmi_copy_calibration(clmi_ctx *ctx, size_t bytes, int iters);
 
 #ifdef __cplusplus
 }
