@@ -150,10 +150,14 @@ def main():
     for name, prec in (("f64", clmi.CLMI_F64), ("f32_bulk", clmi.CLMI_F32_BULK)):
         log(f"[bench] {name}: init {N}^3")
         lat, dx = make_lattice(clmi, spec, N, kIR, dt, prec, args.seed)
+        tune = lat.autotune(5)
+        log(f"[bench] {name}: autotune\n{tune}")
         log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")
         sec, kms, launches, bpl = timed_run(lat, args.warmup, args.steps)
         res[name] = {"seconds": sec, "kernel_ms": kms, "launches": launches, "bytes_per_launch": bpl,
-                     "site_updates_per_s": N ** 3 * args.steps / sec, "ms_per_step": 1e3 * sec / args.steps}
+                     "site_updates_per_s": N ** 3 * args.steps / sec, "ms_per_step": 1e3 * sec / args.steps,
+                     "autotune_us": {ln.rsplit(" ", 1)[0]: float(ln.rsplit(" ", 1)[1])
+                                     for ln in tune.strip().splitlines()}}
         log(f"[bench] {name}: {res[name]['ms_per_step']:.4f} ms/step, "
             f"{res[name]['site_updates_per_s'] / 1e9:.2f} G site-updates/s")
         lats[name] = lat
@@ -179,8 +183,9 @@ def main():
         achieved = r["bytes_per_launch"] / avg_s / 1e9
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_scalar_pass (fused drift+kick+reductions)",
-                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"]}
+                "kernel": "k_scalar_tiled (fused drift+kick+reductions)",
+                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"],
+                "autotune_us": r["autotune_us"]}
 
     if args.calibrate_copy:
         # A plain copy kernel of known size (k_copy): its FETCH_SIZE / WRITE_SIZE counters

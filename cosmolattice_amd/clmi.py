@@ -116,6 +116,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_profile_enable": (C.c_int, [P, C.c_int]),
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
+        "clmi_autotune": (C.c_int, [P, C.c_int, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -390,6 +391,11 @@ class Lattice:
         b = C.c_double()
         self._check(self.lib.clmi_profile_read(self._h, C.byref(ms), C.byref(n), C.byref(b)), "profile_read")
         return ms.value, n.value, b.value
+
+    def autotune(self, reps: int = 5) -> str:
+        buf = C.create_string_buffer(4096)
+        self._check(self.lib.clmi_autotune(self._h, reps, buf, len(buf)), "autotune")
+        return buf.value.decode()
 
     def copy_calibration(self, nbytes: int, iters: int = 3):
         self._check(self.lib.clmi_copy_calibration(self._h, nbytes, iters), "copy_calibration")

@@ -43,7 +43,9 @@ struct clmi_ctx {
   double *partials = nullptr;
   double *dscratch = nullptr;  // small device scratch for row sums
   int blocks_s = 0, blocks_g = 0;
-  int tiled_xc = 0;  // > 0: scalar passes use the tiled kernel with this x-chunk
+  bool tiled = false;  // scalar passes use the tiled kernel with plan tplan
+  TiledPlan tplan;
+  size_t partial_rows = 0;
 
   ScalarPoly spoly{};
   CsPoly cpoly{};
@@ -165,6 +167,30 @@ int prof_event(clmi_ctx *c)
   return CLMI_OK;
 }
 
+ScalarPassArgs scalar_args(const clmi_ctx *c, int fi, int fo, int pi, int po, double cdt_s, double ckick_s,
+                           int row0)
+{
+  ScalarPassArgs a{};
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    a.phi_in[f] = c->fS[f][fi].p;
+    a.philo_in[f] = c->fS[f][fi].lo;
+    a.pi_in[f] = c->pS[f][pi].p;
+    a.phi_out[f] = c->fS[f][fo].p;
+    a.philo_out[f] = c->fS[f][fo].lo;
+    a.pi_out[f] = c->pS[f][po].p;
+  }
+  a.N = c->N;
+  a.V = c->V;
+  a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
+  a.inv_dx = 1.0 / c->cfg.dx;
+  a.cdt = cdt_s;
+  a.ckick = ckick_s;
+  a.st = c->dstate;
+  a.partials = c->partials + (size_t)row0 * kNumRed;
+  a.poly = c->spoly;
+  return a;
+}
+
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
 int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
 {
@@ -172,32 +198,16 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
   if (timed) CHK(prof_event(c));
   int r = 0;
   if (c->cfg.n_scalars > 0) {
-    ScalarPassArgs a{};
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
-    for (int f = 0; f < c->cfg.n_scalars; ++f) {
-      a.phi_in[f] = c->fS[f][fi].p;
-      a.philo_in[f] = c->fS[f][fi].lo;
-      a.pi_in[f] = c->pS[f][pi].p;
-      a.phi_out[f] = c->fS[f][fo].p;
-      a.philo_out[f] = c->fS[f][fo].lo;
-      a.pi_out[f] = c->pS[f][po].p;
-    }
-    a.N = c->N;
-    a.V = c->V;
-    a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
-    a.inv_dx = 1.0 / c->cfg.dx;
-    a.cdt = cdt_s;
-    a.ckick = ckick_s;
-    a.st = c->dstate;
-    a.partials = c->partials + (size_t)r * kNumRed;
-    a.poly = c->spoly;
-    if (c->tiled_xc > 0)
-      HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->tiled_xc,
-                                    c->stream));
-    else
+    const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    if (c->tiled) {
+      HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->tplan, c->stream));
+      if (red) r += c->tplan.rows;
+    } else {
       HIPCHK(c, launch_scalar_pass(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->stream));
-    if (red) r += c->blocks_s;
+      if (red) r += c->blocks_s;
+    }
     c->cur_fS = fo;
     c->cur_pS = po;
   }
@@ -407,14 +417,17 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   c->blocks_s = scalar_pass_blocks(c->V);
   c->blocks_g = gauge_pass_blocks(c->V);
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
-    int tb = 0, xc = 0;
-    scalar_tiled_geometry(g.N, &tb, &xc);
-    if (tb > 0 && tb <= kMaxPassBlocks) {
-      c->blocks_s = tb;
-      c->tiled_xc = xc;
-    }
+    TiledPlan p;
+    for (int xc : {16, 8, 4, 32, 64})
+      if (scalar_tiled_plan(g.N, g.n_scalars, xc, true, &p)) {
+        c->tiled = true;
+        c->tplan = p;
+        break;
+      }
   }
-  const size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
+  size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
+  if (c->tiled && (size_t)c->tplan.rows + c->blocks_g + 16 > rows) rows = (size_t)c->tplan.rows + c->blocks_g + 16;
+  c->partial_rows = rows;
   if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dstate, sizeof(DevState)) != hipSuccess) {
@@ -878,6 +891,58 @@ int clmi_profile_enable(clmi_ctx *c, int on)
   c->prof = on != 0;
   c->prof_ms = 0.0;
   c->prof_launches = 0;
+  return CLMI_OK;
+}
+
+int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
+{
+  CHK(check_ctx(c));
+  if (report && report_len) report[0] = '\0';
+  if (!c->tiled || c->cfg.n_scalars == 0) return CLMI_OK;
+  if (reps < 1) reps = 1;
+  CHK(flush(c));
+  // fused drift+kick+reduce passes from the current buffers into the spare ones; parities are not
+  // flipped and no background update runs, so the context state is unchanged
+  const ScalarPassArgs a =
+      scalar_args(c, c->cur_fS, 1 - c->cur_fS, c->cur_pS, 1 - c->cur_pS, c->cfg.dt, 0.0, 0);
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  std::string rep;
+  double best = 1e300;
+  TiledPlan best_plan = c->tplan;
+  int rc = CLMI_OK;
+  for (int nt = 0; nt < 2 && rc == CLMI_OK; ++nt)
+    for (int xc : {4, 8, 16, 32, 64}) {
+      TiledPlan p;
+      if (!scalar_tiled_plan(c->N, c->cfg.n_scalars, xc, nt != 0, &p)) continue;
+      hipError_t e = launch_scalar_tiled(c->cfg.n_scalars, c->f32, true, true, true, a, p, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int i = 0; i < reps && e == hipSuccess; ++i)
+        e = launch_scalar_tiled(c->cfg.n_scalars, c->f32, true, true, true, a, p, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(c, CLMI_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      const double us = 1e3 * ms / reps;
+      rep += "XC=" + std::to_string(xc) + " NT=" + std::to_string(nt) + " " + std::to_string(us) + "\n";
+      if (us < best) {
+        best = us;
+        best_plan = p;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CLMI_OK) return rc;
+  c->tplan = best_plan;
+  if (report && report_len) {
+    std::strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = '\0';
+  }
   return CLMI_OK;
 }
 

@@ -165,9 +165,16 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
 }
 
 // Launchers (defined in the .hip files).
-void scalar_tiled_geometry(int N, int *blocks, int *XC);
+struct TiledPlan {
+  int XC = 0;      // planes per workgroup (x-chunk)
+  int G = 0;       // planes per partial-sum group
+  bool NT = false; // non-temporal stores
+  int blocks = 0;  // workgroups
+  int rows = 0;    // partial-sum rows written by a reducing pass
+};
+bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                               int blocks, int XC, hipStream_t s);
+                               const TiledPlan &p, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 int scalar_pass_blocks(uint32_t V);
 hipError_t launch_scalar_pass(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,

@@ -130,8 +130,19 @@ struct Tr<true> {
   }
 };
 
-template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY>
-__global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a, const int XC)
+template <bool NT, typename S>
+__device__ __forceinline__ void store_out(S *p, uint32_t i, S v)
+{
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p + i);
+  else
+    p[i] = v;
+}
+
+// Partial sums are written per (group of G planes, y-tile, z-tile), independent of the x-chunk
+// length XC (a multiple of G), so every XC gives the same summation order.
+template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY, bool NT>
+__global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a, const int XC, const int G)
 {
   using T = Tr<F32>;
   using V = typename T::V;
@@ -243,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
         const V vxp = nxt[r][f], vxm = prev[r][f];
         if constexpr (!F32) {
           phid[f] = c;
-          if (DRIFT) ((double *)a.phi_out[f])[s] = c;
+          if (DRIFT) store_out<NT>((double *)a.phi_out[f], s, c);
           if (KICK) {
             double l = 0.0;
             l += vxp + vxm - 2 * c;
@@ -261,8 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
           phif[f] = c.x + c.y;
           phid[f] = (double)c.x + (double)c.y;
           if (DRIFT) {
-            ((float *)a.phi_out[f])[s] = c.x;
-            a.philo_out[f][s] = c.y;
+            store_out<NT>((float *)a.phi_out[f], s, c.x);
+            store_out<NT>(a.philo_out[f], s, c.y);
           }
           const float dxp = (vxp.x - c.x) + (vxp.y - c.y);
           const float dyp = (vyp.x - c.x) + (vyp.y - c.y);
@@ -288,14 +299,14 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
           double p = pic[r][f];
           if (KICK) {
             p += a.ckick * (aK * lapd[f] - aV * dpot_d<NF>(a.poly, phid, f));
-            ((double *)a.pi_out[f])[s] = p;
+            store_out<NT>((double *)a.pi_out[f], s, p);
           }
           p2 += p * p;
         } else {
           float p = pic[r][f];
           if (KICK) {
             p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
-            ((float *)a.pi_out[f])[s] = p;
+            store_out<NT>((float *)a.pi_out[f], s, p);
           }
           p2 += (double)p * (double)p;
         }
@@ -305,6 +316,15 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
         acc1 += g2;
         acc2 += pot_d<NF>(a.poly, phid);
       }
+    }
+
+    if (RED && (i + 1) % G == 0) {
+      // one partial row per (plane group, y-tile, z-tile); the loop barrier below separates
+      // consecutive uses of the reduction's shared scratch
+      double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
+      const size_t row = ((size_t)(x / G) * nTy + ty) * nTz + tz;
+      block_reduce_store<kNumRed>(v, a.partials + row * kNumRed);
+      acc0 = acc1 = acc2 = 0.0;
     }
 
     if (more) {
@@ -331,23 +351,17 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
       __syncthreads();
     }
   }
-
-  if (RED) {
-    double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
-    __syncthreads();
-    block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
-  }
 }
 
-template <int NF, bool F32>
-hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC,
+template <int NF, bool F32, bool NT>
+hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
                           hipStream_t s)
 {
   constexpr int RY = 4;
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
-#define CLMI_ST(D, K, R)                                                                                     \
-  case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                           \
-    hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY>), dim3(blocks), dim3(kBlock), 0, s, a, XC);     \
+#define CLMI_ST(D, K, R)                                                                                        \
+  case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                              \
+    hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY, NT>), dim3(blocks), dim3(kBlock), 0, s, a, XC, G); \
     break;
   switch (code) {
     CLMI_ST(true, true, true)
@@ -366,34 +380,36 @@ hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs 
 
 }  // namespace
 
-// Tile geometry for N (0 blocks: the tiled pass does not apply and the generic pass is used).
-void scalar_tiled_geometry(int N, int *blocks, int *XC)
+// Tile plan for N with x-chunk XC (false: the tiled pass does not apply).  G, the plane group of
+// the partial sums, depends on N only.
+bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 {
   constexpr int TY = 16;
-  *blocks = 0;
-  *XC = 0;
-  if (N % TZ != 0 || N % TY != 0) return;
+  if (nf < 1 || nf > 2 || N % TZ != 0 || N % TY != 0) return false;
+  const int G = N <= 256 ? 4 : N / 64;
+  if (XC < G || XC % G != 0 || N % XC != 0) return false;
   const long long tiles = (long long)(N / TZ) * (N / TY);
-  int xc = 32;
-  while (xc > 4 && tiles * (N / xc) < 1024) xc /= 2;
-  while (N % xc != 0) xc /= 2;
-  if (xc < 2) return;
-  *XC = xc;
-  *blocks = (int)(tiles * (N / xc));
+  p->XC = XC;
+  p->G = G;
+  p->NT = nt;
+  p->blocks = (int)(tiles * (N / XC));
+  p->rows = (int)(tiles * (N / G));
+  return true;
 }
 
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                               int blocks, int XC, hipStream_t s)
+                               const TiledPlan &p, hipStream_t s)
 {
-  switch (nf * 2 + (f32 ? 1 : 0)) {
-    case 2: return dispatch_tiled<1, false>(drift, kick, red, a, blocks, XC, s);
-    case 3: return dispatch_tiled<1, true>(drift, kick, red, a, blocks, XC, s);
-    case 4: return dispatch_tiled<2, false>(drift, kick, red, a, blocks, XC, s);
-    case 5: return dispatch_tiled<2, true>(drift, kick, red, a, blocks, XC, s);
-    case 6: return dispatch_tiled<3, false>(drift, kick, red, a, blocks, XC, s);
-    case 7: return dispatch_tiled<3, true>(drift, kick, red, a, blocks, XC, s);
-    case 8: return dispatch_tiled<4, false>(drift, kick, red, a, blocks, XC, s);
-    case 9: return dispatch_tiled<4, true>(drift, kick, red, a, blocks, XC, s);
+  const int code = (nf == 2 ? 4 : 0) | (f32 ? 2 : 0) | (p.NT ? 1 : 0);
+  switch (code) {
+    case 0: return dispatch_tiled<1, false, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 1: return dispatch_tiled<1, false, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 2: return dispatch_tiled<1, true, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 3: return dispatch_tiled<1, true, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 4: return dispatch_tiled<2, false, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 5: return dispatch_tiled<2, false, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 6: return dispatch_tiled<2, true, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 7: return dispatch_tiled<2, true, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
     default: return hipErrorInvalidValue;
   }
 }

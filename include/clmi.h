@@ -189,7 +189,15 @@ int clmi_spectrum_bins(clmi_ctx *ctx, int kind, int index, int comp, double dk, 
 int clmi_grf_scalar(clmi_ctx *ctx, int index, uint64_t seed, uint32_t stream, double mass2, double aDot,
                     double kIR, double kCutOff, double fld0, double pi0);
 
-/* ---- timing --------------------------------------------------------------------------------- */
+/* ---- tuning and timing ------------------------------------------------------------------------ */
+
+/* No counterpart in the reference (tile shapes belong to this library's kernels).  Times every
+ * candidate plan of the tiled lattice pass -- x-chunk length and store mode -- with `reps` fused
+ * passes on the context's own data, without changing the context's fields or background, keeps
+ * the fastest and writes one "XC=<n> NT=<0|1> <microseconds>" line per candidate into report
+ * (if non-NULL).  All candidates give bit-identical results, including the volume sums.  A no-op
+ * returning CLMI_OK where the tiled pass does not apply. */
+int clmi_autotune(clmi_ctx *ctx, int reps, char *report, size_t report_len);
 
 /* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */
 int clmi_profile_enable(clmi_ctx *ctx, int on);

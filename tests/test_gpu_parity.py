@@ -141,6 +141,39 @@ def test_tiled_pass_bit_identical_to_generic(gpu, N, prec):
         assert res[0][1][k] == pytest.approx(res[1][1][k], rel=1e-13)
 
 
+@pytest.mark.parametrize("prec", [CLMI_F64, CLMI_F32_BULK])
+def test_autotune_keeps_state_and_results(gpu, prec):
+    """Autotuning runs passes on the live data but leaves fields and background untouched, and the
+    plan it picks gives bit-identical results (volume sums included) to the default plan."""
+    say(f"case autotune prec={prec}")
+    spec = bench_spec()
+    N = 128
+    rng = np.random.default_rng(77)
+    ics = scalar_ics(spec, N, rng)
+    outs = []
+    for tune in (False, True):
+        lat = Lattice(spec, N, 0.4909, 0.005, prec)
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.set_background(aDotI=0.6)
+        lat.lf_init()
+        lat.lf_evolve(7)
+        if tune:
+            before = [lat.download(k, i, 0) for (k, i) in sorted(ics)]
+            bg_before = lat.background()
+            rep = lat.autotune(2)
+            say(rep)
+            assert "XC=" in rep
+            assert all(np.array_equal(x, lat.download(k, i, 0)) for x, (k, i) in zip(before, sorted(ics)))
+            assert lat.background() == bg_before
+        lat.lf_evolve(13)
+        outs.append(([lat.download(k, i, 0) for (k, i) in sorted(ics)], lat.background()))
+        lat.close()
+    for a, b in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(a, b)
+    assert outs[0][1] == outs[1][1]
+
+
 def test_lf_lphi4_fp64_parity_64_tiled(gpu, oracle_mod):
     """64^3 (tiled pass) against the oracle: 100 leapfrog steps, measurements every 25."""
     say("case lf_lphi4_fp64_64")
