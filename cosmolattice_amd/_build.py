@@ -23,6 +23,7 @@ OBJDIR = PKG / "_objs"
 HIP_SOURCES = [
     "scalar_pass.hip",
     "scalar_tiled.hip",
+    "scalar_rows.hip",
     "gauge_pass.hip",
     "background.hip",
     "fft_kernels.hip",

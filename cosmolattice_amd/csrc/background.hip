@@ -33,35 +33,61 @@ __device__ double sf_kernel(const clmi_background &b, const ModelConsts &mc)
           (mc.alpha + 1) * b.potAvI);
 }
 
-__global__ __launch_bounds__(256) void k_update(DevState *st, const double *partials, int rows, ModelConsts mc,
-                                               UpdateOps ops)
+constexpr int kUpdThreads = 1024;
+
+// Fixed-order sum of `rows` partial rows: thread t takes rows t, t + 1024, ... (four loads in
+// flight per slot), then wavefront shuffles, then a sum over the 16 wavefronts in order.
+__device__ void sum_partials(const double *partials, int rows, double *out /* kNumRed, in LDS */)
 {
-  __shared__ double sh[256][kNumRed];
+  __shared__ double sh[kUpdThreads / 64][kNumRed];
+  const int t = threadIdx.x;
+  double v[kNumRed];
+#pragma unroll
+  for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
+  int r = t;
+  for (; r + 3 * kUpdThreads < rows; r += 4 * kUpdThreads) {
+    double q[4][kNumRed];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < kNumRed; ++k) q[u][k] = partials[(size_t)(r + u * kUpdThreads) * kNumRed + k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < kNumRed; ++k) v[k] += q[u][k];
+  }
+  for (; r < rows; r += kUpdThreads)
+#pragma unroll
+    for (int k = 0; k < kNumRed; ++k) v[k] += partials[(size_t)r * kNumRed + k];
+#pragma unroll
+  for (int k = 0; k < kNumRed; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < kNumRed; ++k) sh[t >> 6][k] = v[k];
+  __syncthreads();
+  if (t < kNumRed) {
+    double s = 0.0;
+    for (int w = 0; w < kUpdThreads / 64; ++w) s += sh[w][t];
+    out[t] = s;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kUpdThreads) void k_update(DevState *st, const double *partials, int rows,
+                                                        ModelConsts mc, UpdateOps ops)
+{
+  __shared__ double tot[kNumRed];
   const int t = threadIdx.x;
   if (rows > 0) {
-    double v[kNumRed];
-#pragma unroll
-    for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
-    for (int r = t; r < rows; r += 256)
-#pragma unroll
-      for (int k = 0; k < kNumRed; ++k) v[k] += partials[(size_t)r * kNumRed + k];
-#pragma unroll
-    for (int k = 0; k < kNumRed; ++k) sh[t][k] = v[k];
-    __syncthreads();
-    for (int stride = 128; stride > 0; stride >>= 1) {
-      if (t < stride)
-#pragma unroll
-        for (int k = 0; k < kNumRed; ++k) sh[t][k] += sh[t + stride][k];
-      __syncthreads();
-    }
-    if (t == 0)
-#pragma unroll
-      for (int k = 0; k < kNumRed; ++k) st->red[k] = sh[0][k] * mc.invV;
+    sum_partials(partials, rows, tot);
+    if (t < kNumRed) st->red[t] = tot[t] * mc.invV;
   }
   if (t != 0) return;
 
   clmi_background &b = st->bg;
-  const double *red = st->red;
+  double red[kNumRed];
+  for (int k = 0; k < kNumRed; ++k) red[k] = rows > 0 ? tot[k] * mc.invV : st->red[k];
   const double dt = mc.dt;
   for (int o = 0; o < ops.n; ++o) {
     const double w = ops.w[o];
@@ -216,26 +242,11 @@ __global__ __launch_bounds__(kBlock) void k_means(const MeansArgs a)
   block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
 }
 
-__global__ __launch_bounds__(256) void k_sum_rows(const double *partials, int rows, double *out)
+__global__ __launch_bounds__(kUpdThreads) void k_sum_rows(const double *partials, int rows, double *out)
 {
-  __shared__ double sh[256][kNumRed];
-  const int t = threadIdx.x;
-  double v[kNumRed];
-#pragma unroll
-  for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
-  for (int r = t; r < rows; r += 256)
-#pragma unroll
-    for (int k = 0; k < kNumRed; ++k) v[k] += partials[(size_t)r * kNumRed + k];
-#pragma unroll
-  for (int k = 0; k < kNumRed; ++k) sh[t][k] = v[k];
-  __syncthreads();
-  for (int stride = 128; stride > 0; stride >>= 1) {
-    if (t < stride)
-#pragma unroll
-      for (int k = 0; k < kNumRed; ++k) sh[t][k] += sh[t + stride][k];
-    __syncthreads();
-  }
-  if (t < kNumRed) out[t] = sh[0][t];
+  __shared__ double tot[kNumRed];
+  sum_partials(partials, rows, tot);
+  if (threadIdx.x < kNumRed) out[threadIdx.x] = tot[threadIdx.x];
 }
 
 __global__ void k_split(const double *src, float *hi, float *lo, uint32_t V)
@@ -274,7 +285,7 @@ int conv_blocks(uint32_t V)
 hipError_t launch_update(DevState *st, const double *partials, int rows, const ModelConsts &mc,
                          const UpdateOps &ops, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_update, dim3(1), dim3(256), 0, s, st, partials, rows, mc, ops);
+  hipLaunchKernelGGL(k_update, dim3(1), dim3(kUpdThreads), 0, s, st, partials, rows, mc, ops);
   return hipGetLastError();
 }
 
@@ -298,7 +309,7 @@ hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s)
 
 hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, s, partials, rows, out);
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kUpdThreads), 0, s, partials, rows, out);
   return hipGetLastError();
 }
 

@@ -45,6 +45,8 @@ struct clmi_ctx {
   int blocks_s = 0, blocks_g = 0;
   bool tiled = false;  // scalar passes use the tiled kernel with plan tplan
   TiledPlan tplan;
+  bool rowk = false;   // scalar passes use the full-row kernel with plan rplan
+  RowsPlan rplan;
   size_t partial_rows = 0;
 
   ScalarPoly spoly{};
@@ -201,7 +203,10 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
     const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
-    if (c->tiled) {
+    if (c->rowk) {
+      HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream));
+      if (red) r += c->rplan.rows;
+    } else if (c->tiled) {
       HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->tplan, c->stream));
       if (red) r += c->tplan.rows;
     } else {
@@ -417,13 +422,22 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   c->blocks_s = scalar_pass_blocks(c->V);
   c->blocks_g = gauge_pass_blocks(c->V);
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
-    TiledPlan p;
-    for (int xc : {16, 8, 4, 32, 64})
-      if (scalar_tiled_plan(g.N, g.n_scalars, xc, true, &p)) {
-        c->tiled = true;
-        c->tplan = p;
+    RowsPlan rp;
+    for (int xc : {32, 16, 8, 4})
+      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, true,
+                           (g.flags & CLMI_FLAG_SHAPE_ALT) != 0, &rp)) {
+        c->rowk = true;
+        c->rplan = rp;
         break;
       }
+    TiledPlan p;
+    if (!c->rowk)
+      for (int xc : {16, 8, 4, 32, 64})
+        if (scalar_tiled_plan(g.N, g.n_scalars, xc, true, &p)) {
+          c->tiled = true;
+          c->tplan = p;
+          break;
+        }
   }
   size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
   if (c->tiled && (size_t)c->tplan.rows + c->blocks_g + 16 > rows) rows = (size_t)c->tplan.rows + c->blocks_g + 16;
@@ -898,7 +912,7 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
 {
   CHK(check_ctx(c));
   if (report && report_len) report[0] = '\0';
-  if (!c->tiled || c->cfg.n_scalars == 0) return CLMI_OK;
+  if ((!c->tiled && !c->rowk) || c->cfg.n_scalars == 0) return CLMI_OK;
   if (reps < 1) reps = 1;
   CHK(flush(c));
   // fused drift+kick+reduce passes from the current buffers into the spare ones; parities are not
@@ -911,15 +925,23 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   std::string rep;
   double best = 1e300;
   TiledPlan best_plan = c->tplan;
+  RowsPlan best_rplan = c->rplan;
   int rc = CLMI_OK;
   for (int nt = 0; nt < 2 && rc == CLMI_OK; ++nt)
     for (int xc : {4, 8, 16, 32, 64}) {
       TiledPlan p;
-      if (!scalar_tiled_plan(c->N, c->cfg.n_scalars, xc, nt != 0, &p)) continue;
-      hipError_t e = launch_scalar_tiled(c->cfg.n_scalars, c->f32, true, true, true, a, p, c->stream);
+      RowsPlan rp;
+      if (c->rowk ? !scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, nt != 0,
+                                      (c->cfg.flags & CLMI_FLAG_SHAPE_ALT) != 0, &rp)
+                  : !scalar_tiled_plan(c->N, c->cfg.n_scalars, xc, nt != 0, &p))
+        continue;
+      auto launch = [&]() {
+        return c->rowk ? launch_scalar_rows(c->cfg.n_scalars, c->f32, true, true, true, a, rp, c->stream)
+                       : launch_scalar_tiled(c->cfg.n_scalars, c->f32, true, true, true, a, p, c->stream);
+      };
+      hipError_t e = launch();
       if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-      for (int i = 0; i < reps && e == hipSuccess; ++i)
-        e = launch_scalar_tiled(c->cfg.n_scalars, c->f32, true, true, true, a, p, c->stream);
+      for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
       if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.0f;
@@ -933,12 +955,16 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
       if (us < best) {
         best = us;
         best_plan = p;
+        best_rplan = rp;
       }
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CLMI_OK) return rc;
-  c->tplan = best_plan;
+  if (c->rowk)
+    c->rplan = best_rplan;
+  else
+    c->tplan = best_plan;
   if (report && report_len) {
     std::strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = '\0';

@@ -173,6 +173,15 @@ struct TiledPlan {
   int rows = 0;    // partial-sum rows written by a reducing pass
 };
 bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p);
+struct RowsPlan {
+  int VZ = 0, WPR = 0, TY = 0;  // sites per lane, wavefronts per z-row, rows per workgroup
+  int XC = 0, G = 0;
+  bool NT = false;
+  int blocks = 0, rows = 0;
+};
+bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, bool alt, RowsPlan *p);
+hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
+                              const RowsPlan &p, hipStream_t s);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                                const TiledPlan &p, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);

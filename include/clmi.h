@@ -42,6 +42,10 @@ extern "C" {
 /* clmi_config.flags: use the generic one-site-per-thread lattice pass even where the tiled
  * (x-marching, LDS) pass applies.  Both give bit-identical results; the flag exists for testing. */
 #define CLMI_FLAG_NO_TILED 1
+/* clmi_config.flags: the alternative workgroup shape of the full-row lattice pass (N = 256).  Fields
+ * are bit-identical to the default shape; volume sums differ in summation order only.  Used to
+ * compare shapes. */
+#define CLMI_FLAG_SHAPE_ALT 2
 
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
 #define CLMI_FLD_S 0

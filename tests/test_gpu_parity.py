@@ -111,19 +111,23 @@ def test_lf_lphi4_fp64_parity_100_steps(gpu, oracle_mod, N):
     assert bg["aI"] > 1.0
 
 
-@pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (64, CLMI_F32_BULK), (128, CLMI_F32_BULK), (128, CLMI_F64)])
-def test_tiled_pass_bit_identical_to_generic(gpu, N, prec):
-    """The tiled x-marching pass (used for N % 64 == 0) and the generic one-site-per-thread pass
-    give bit-identical fields and momenta after 40 leapfrog steps on a static background (the
-    per-site arithmetic is the same); their volume sums differ only in summation order."""
+@pytest.mark.parametrize("N,prec,flags", [
+    (64, CLMI_F64, 0), (64, CLMI_F32_BULK, 0), (128, CLMI_F32_BULK, 0), (128, CLMI_F64, 0),
+    (192, CLMI_F32_BULK, 0), (192, CLMI_F64, 0), (256, CLMI_F32_BULK, 0), (256, CLMI_F64, 0),
+    (256, CLMI_F32_BULK, 2), (256, CLMI_F64, 2), (320, CLMI_F32_BULK, 0), (320, CLMI_F64, 0)])
+def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
+    """The full-row pass (N = 64, 128, 192, 256, both workgroup shapes) and the tiled x-marching
+    pass (other multiples of 64) give fields and momenta bit-identical to the generic
+    one-site-per-thread pass after 40 leapfrog steps on a static background (same per-site
+    arithmetic); their volume sums differ only in summation order."""
     from cosmolattice_amd.clmi import CLMI_FLAG_NO_TILED
 
-    say(f"case tiled_vs_generic N={N} prec={prec}")
+    say(f"case fast_vs_generic N={N} prec={prec} flags={flags}")
     spec = bench_spec()
     rng = np.random.default_rng(N + prec)
     ics = scalar_ics(spec, N, rng)
     res = []
-    for flags in (0, CLMI_FLAG_NO_TILED):
+    for flags in (flags, CLMI_FLAG_NO_TILED):
         lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
         for (k, i), a in ics.items():
             lat.upload(k, i, 0, a)
