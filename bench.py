@@ -37,9 +37,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_lattice(clmi, spec, N, kIR, dt, precision, seed):
+def make_lattice(clmi, spec, N, kIR, dt, precision, seed, flags=0):
     dx = 2 * math.pi / kIR / N
-    lat = clmi.Lattice(spec, N, dx, dt, precision, expansion=True)
+    lat = clmi.Lattice(spec, N, dx, dt, precision, expansion=True, flags=flags)
     # ScaleFactorInitializer (initializers/scalefactorinitializer.h:49-64): aDot from the
     # homogeneous energy, then the fluctuations are added with that aDot
     phi0 = spec.fld0
@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--flags", type=int, default=0, help="clmi_config.flags for both lattices")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -149,7 +150,7 @@ def main():
     lats = {}
     for name, prec in (("f64", clmi.CLMI_F64), ("f32_bulk", clmi.CLMI_F32_BULK)):
         log(f"[bench] {name}: init {N}^3")
-        lat, dx = make_lattice(clmi, spec, N, kIR, dt, prec, args.seed)
+        lat, dx = make_lattice(clmi, spec, N, kIR, dt, prec, args.seed, args.flags)
         tune = lat.autotune(5)
         log(f"[bench] {name}: autotune\n{tune}")
         log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")

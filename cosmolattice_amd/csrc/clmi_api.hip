@@ -194,15 +194,26 @@ ScalarPassArgs scalar_args(const clmi_ctx *c, int fi, int fo, int pi, int po, do
 }
 
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
-int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
+// With fold_ops non-null and a single-sector model on the full-row kernel, the pass also finishes
+// the background update with those operations in its last workgroup (*folded = true).
+int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows,
+             const UpdateOps *fold_ops = nullptr, bool *folded = nullptr)
 {
   const bool timed = c->prof && drift && kick;
   if (timed) CHK(prof_event(c));
   int r = 0;
+  if (folded) *folded = false;
   if (c->cfg.n_scalars > 0) {
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
-    const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    if (c->rowk && red && fold_ops && folded && c->cfg.n_cscalars == 0) {
+      a.fold = 1;
+      a.fold_rows = c->rplan.rows;
+      a.mc = c->mc;
+      a.ops = *fold_ops;
+      *folded = true;
+    }
     if (c->rowk) {
       HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream));
       if (red) r += c->rplan.rows;
@@ -261,7 +272,7 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
   return CLMI_OK;
 }
 
-int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> ops)
+UpdateOps make_ops(std::initializer_list<std::pair<int, double>> ops)
 {
   UpdateOps u{};
   for (const auto &o : ops) {
@@ -269,7 +280,25 @@ int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> 
     u.w[u.n] = o.second;
     u.n++;
   }
-  HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
+  return u;
+}
+
+int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> ops)
+{
+  HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, make_ops(ops), c->stream));
+  return CLMI_OK;
+}
+
+// A lattice pass followed by the background update `ops` (skipped when do_update is false):
+// folded into the pass where possible, otherwise a separate update launch.
+int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g,
+                bool do_update, std::initializer_list<std::pair<int, double>> ops)
+{
+  const UpdateOps u = make_ops(ops);
+  int rows = 0;
+  bool folded = false;
+  CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows, do_update ? &u : nullptr, &folded));
+  if (do_update && !folded) HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
   return CLMI_OK;
 }
 
@@ -279,8 +308,8 @@ int flush(clmi_ctx *c)
   if (!c->pending_drift) return CLMI_OK;
   const bool ex = c->cfg.expansion != 0;
   int rows = 0;
-  CHK(run_pass(c, true, false, ex, c->cfg.dt, 0.0, 0.0, &rows));
-  if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}}));
+  (void)rows;
+  CHK(pass_update(c, true, false, ex, c->cfg.dt, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
   c->pending_drift = false;
   return CLMI_OK;
 }
@@ -613,12 +642,11 @@ int clmi_lf_evolve(clmi_ctx *c, int nsteps)
     const double w = c->synced ? 0.5 : 1.0;
     int rows = 0;
     if (c->pending_drift) {
-      CHK(run_pass(c, true, true, ex, dt, w * dt, w * dt, &rows));
-      if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}));
+      CHK(pass_update(c, true, true, ex, dt, w * dt, w * dt, ex, {{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}));
     } else {
-      CHK(run_pass(c, false, true, ex, 0.0, w * dt, w * dt, &rows));
-      if (ex) CHK(update(c, rows, {{OP_LF_MOMENTA, 0.0}}));
+      CHK(pass_update(c, false, true, ex, 0.0, w * dt, w * dt, ex, {{OP_LF_MOMENTA, 0.0}}));
     }
+    (void)rows;
     c->pending_drift = true;
     c->synced = false;
   }
@@ -632,8 +660,8 @@ int clmi_lf_sync(clmi_ctx *c)
   if (!c->synced) {
     const bool ex = c->cfg.expansion != 0;
     int rows = 0;
-    CHK(run_pass(c, false, true, ex, 0.0, 0.5 * c->cfg.dt, 0.5 * c->cfg.dt, &rows));
-    if (ex) CHK(update(c, rows, {{OP_LF_SYNC, 0.0}}));
+    (void)rows;
+    CHK(pass_update(c, false, true, ex, 0.0, 0.5 * c->cfg.dt, 0.5 * c->cfg.dt, ex, {{OP_LF_SYNC, 0.0}}));
   }
   c->synced = true;
   return CLMI_OK;
@@ -654,12 +682,11 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
       const double w = ws[s < nws ? s : stages - s - 1];
       int rows = 0;
       if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
-      CHK(run_pass(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, &rows));
-      if (ex) CHK(update(c, rows, {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
-      CHK(run_pass(c, true, false, ex, dt * w, 0.0, 0.0, &rows));
-      if (ex) CHK(update(c, rows, {{OP_STORE_FIELDS, 0.0}}));
-      CHK(run_pass(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, &rows));
-      if (ex) CHK(update(c, rows, {{OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
+      (void)rows;
+      CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex,
+                      {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
+      CHK(pass_update(c, true, false, ex, dt * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
+      CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex, {{OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
     }
   return CLMI_OK;
 }
@@ -687,8 +714,8 @@ int clmi_set_all_averages(clmi_ctx *c)
   CHK(check_ctx(c));
   CHK(flush(c));
   int rows = 0;
-  CHK(run_pass(c, false, false, true, 0.0, 0.0, 0.0, &rows));
-  CHK(update(c, rows, {{OP_SET_ALL, 0.0}}));
+  (void)rows;
+  CHK(pass_update(c, false, false, true, 0.0, 0.0, 0.0, true, {{OP_SET_ALL, 0.0}}));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLMI_OK;
 }
@@ -699,8 +726,8 @@ int clmi_volume_averages(clmi_ctx *c, clmi_averages *out)
   if (!out) return fail(c, CLMI_E_INVALID, "null output");
   CHK(flush(c));
   int rows = 0;
-  CHK(run_pass(c, false, false, true, 0.0, 0.0, 0.0, &rows));
-  CHK(update(c, rows, {}));
+  (void)rows;
+  CHK(pass_update(c, false, false, true, 0.0, 0.0, 0.0, true, {}));
   DevState h{};
   CHK(read_state(c, &h));
   out->pi2S = h.red[RED_PI2S];

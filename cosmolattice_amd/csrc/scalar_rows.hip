@@ -11,6 +11,7 @@
 //
 // Volume sums are written per (group of G planes, TY-row y-tile), independent of XC.
 #include "clmi_internal.hpp"
+#include "update_device.hpp"
 
 namespace clmi {
 namespace {
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
   using Raw = typename T::template Raw<VZ>;
   constexpr int NW = WPR * TY;  // wavefronts per workgroup
   constexpr int N = 64 * VZ * WPR;
-  __shared__ V lds[NF][TY + 2][N];
+  __shared__ V ldsbuf[2][NF][TY + 2][N];  // tile of plane x in buffer (x - x0) & 1
 
   const uint32_t NN = (uint32_t)N * N;
   const int nTy = N / TY;
@@ -241,6 +242,8 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
   Raw pf[NF], hq[NF];
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
 
+  {
+  auto &lds = ldsbuf[0];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     Raw q;
@@ -266,11 +269,13 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
       for (int k = 0; k < VZ; ++k) lds[f][hl][zb + k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
     }
   }
+  }
   __syncthreads();
 
   for (int i = 0; i < XC; ++i) {
     const int x = x0 + i;
     const bool more = i + 1 < XC;
+    auto &lds = ldsbuf[i & 1];
     if (more) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -385,7 +390,9 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
       acc0 = acc1 = acc2 = 0.0;
     }
     if (more) {
-      __syncthreads();  // everyone is done with the LDS tile of plane x
+      // the other buffer was last read in the previous iteration, which every wavefront finished
+      // before the barrier that ended it
+      auto &ldn = ldsbuf[(i + 1) & 1];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
 #pragma unroll
@@ -395,13 +402,35 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
           pic[f][k] = pin[f][k];
           nxt[f][k] = T::template value<DRIFT>(pf[f], k, fs, a.cdt, cf);
           pin[f][k] = pf[f].p[k];
-          lds[f][row + 1][zb + k] = cur[f][k];
+          ldn[f][row + 1][zb + k] = cur[f][k];
         }
         if (halo)
 #pragma unroll
-          for (int k = 0; k < VZ; ++k) lds[f][hl][zb + k] = T::template value<DRIFT>(hq[f], k, fs, a.cdt, cf);
+          for (int k = 0; k < VZ; ++k) ldn[f][hl][zb + k] = T::template value<DRIFT>(hq[f], k, fs, a.cdt, cf);
       }
       __syncthreads();  // LDS tile of plane x+1 ready
+    }
+  }
+
+  if (RED && a.fold) {
+    // finish the background update in the last workgroup to complete
+    __shared__ unsigned s_last;
+    __shared__ double tot[kNumRed];
+    __threadfence();
+    __syncthreads();
+    DevState *st = const_cast<DevState *>(a.st);
+    if (threadIdx.x == 0) s_last = atomicAdd(&st->done_count, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      sum_partials<64 * NW>(a.partials, a.fold_rows, tot);
+      if (threadIdx.x < kNumRed) st->red[threadIdx.x] = tot[threadIdx.x] * a.mc.invV;
+      if (threadIdx.x == 0) {
+        double red[kNumRed];
+        for (int k = 0; k < kNumRed; ++k) red[k] = tot[k] * a.mc.invV;
+        apply_update(st, red, a.mc, a.ops);
+        st->done_count = 0;
+      }
     }
   }
 }
@@ -440,8 +469,8 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, bool alt, RowsPl
 {
   if (nf != 2 && nf != 1) return false;
   int vz = 0, wpr = 0, ty = 0;
-  if (N == 256 && !alt) { vz = f32 ? 4 : 2; wpr = f32 ? 1 : 2; ty = 8; }
-  else if (N == 256) { vz = f32 ? 2 : 4; wpr = f32 ? 2 : 1; ty = 8; }
+  if (N == 256 && !alt) { vz = 2; wpr = 2; ty = 8; }
+  else if (N == 256) { vz = 4; wpr = 1; ty = 8; }
   else if (N == 128) { vz = 2; wpr = 1; ty = 8; }
   else if (N == 64) { vz = 1; wpr = 1; ty = 8; }
   else if (N == 192) { vz = 3; wpr = 1; ty = 8; }

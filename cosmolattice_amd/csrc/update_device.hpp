@@ -1,0 +1,182 @@
+// Device-side background update shared by the stand-alone update kernel and by lattice passes
+// that finish the update in their last workgroup.
+//
+// sum_partials: fixed-order sum of the per-workgroup partial rows (bit-reproducible for a given
+// block size).  apply_update: one thread advances the scale factor exactly as the reference's host
+// code does and refreshes the kick/drift coefficients the next pass reads.
+#pragma once
+
+#include "clmi_internal.hpp"
+
+namespace clmi {
+
+// ScaleFactorKernels::get (evolvers/kernels/scalefactorkernels.h:27-81) with the energy
+// normalisations of definitions/energies.h:37-85.
+__device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &mc)
+{
+  const double a = b.aI;
+  double Eks = 0, Egs = 0, Ekcs = 0, Egcs = 0, Eel = 0, Emag = 0;
+  if (mc.ns > 0) {
+    Eks = 0.5 * b.pi2AvI * pow(a, -6.0);
+    Egs = 0.5 * b.grad2AvI * pow(a, -2.0);
+  }
+  if (mc.ncs > 0) {
+    Ekcs = b.CSpi2AvI * pow(a, -6.0);
+    Egcs = b.CSgrad2AvI * pow(a, -2.0);
+  }
+  if (mc.nu1 > 0) {
+    const double w2 = (mc.omegaStar / mc.fStar) * (mc.omegaStar / mc.fStar);
+    Eel = 0.5 * w2 * pow(a, -4.0) * b.U1pi2AvI;
+    Emag = 0.5 * w2 * pow(a, -4.0) * b.U1Mag2AvI;
+  }
+  const double fm = mc.fStar / mc.MPl;
+  return pow(a, 2 * mc.alpha + 1) / 3.0 * (fm * fm) *
+         ((mc.alpha - 2) * (Eks + Ekcs) + mc.alpha * (Egs + Egcs) + (mc.alpha - 1) * (Eel + Emag) +
+          (mc.alpha + 1) * b.potAvI);
+}
+
+// Fixed-order sum of `rows` partial rows by a block of NT threads: thread t takes rows t, t + NT,
+// ... (four loads in flight per slot), then wavefront shuffles, then the wavefront sums in order.
+// Loads are volatile so that rows written by other workgroups of the same launch are read from
+// L2.  Result in out[0..kNumRed) (shared memory); ends with a barrier.
+template <int NT>
+__device__ inline void sum_partials(const double *partials_, int rows, double *out)
+{
+  const volatile double *partials = partials_;
+  __shared__ double sh[NT / 64][kNumRed];
+  const int t = threadIdx.x;
+  double v[kNumRed];
+#pragma unroll
+  for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
+  int r = t;
+  for (; r + 3 * NT < rows; r += 4 * NT) {
+    double q[4][kNumRed];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < kNumRed; ++k) q[u][k] = partials[(size_t)(r + u * NT) * kNumRed + k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < kNumRed; ++k) v[k] += q[u][k];
+  }
+  for (; r < rows; r += NT)
+#pragma unroll
+    for (int k = 0; k < kNumRed; ++k) v[k] += partials[(size_t)r * kNumRed + k];
+#pragma unroll
+  for (int k = 0; k < kNumRed; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < kNumRed; ++k) sh[t >> 6][k] = v[k];
+  __syncthreads();
+  if (t < kNumRed) {
+    double s = 0.0;
+    for (int w = 0; w < NT / 64; ++w) s += sh[w][t];
+    out[t] = s;
+  }
+  __syncthreads();
+}
+
+// Executed by one thread.  red: finished volume averages (sums / N^3) of the last pass.
+__device__ inline void apply_update(DevState *st, const double *red, const ModelConsts &mc, const UpdateOps &ops)
+{
+  clmi_background &b = st->bg;
+  const double dt = mc.dt;
+  for (int o = 0; o < ops.n; ++o) {
+    const double w = ops.w[o];
+    switch (ops.op[o]) {
+      case OP_STORE_FIELDS:  // leapfrog.h:269-277
+        if (mc.ns > 0) b.grad2AvI = red[RED_GRAD2S];
+        if (mc.ncs > 0) b.CSgrad2AvI = red[RED_GRAD2CS];
+        if (mc.nu1 > 0) b.U1Mag2AvI = red[RED_B2U1];
+        b.potAvI = red[RED_POT];
+        break;
+      case OP_LF_MOMENTA:  // leapfrog.h:239-267 then 124-129 and 168-184
+        if (mc.ns > 0) {
+          b.pi2AvSIM = b.pi2AvSI;
+          b.pi2AvSI = red[RED_PI2S];
+          b.pi2AvI = 0.5 * (b.pi2AvSIM + b.pi2AvSI);
+        }
+        if (mc.ncs > 0) {
+          b.CSpi2AvSIM = b.CSpi2AvSI;
+          b.CSpi2AvSI = red[RED_PI2CS];
+          b.CSpi2AvI = 0.5 * (b.CSpi2AvSIM + b.CSpi2AvSI);
+        }
+        if (mc.nu1 > 0) {
+          b.U1pi2AvSIM = b.U1pi2AvSI;
+          b.U1pi2AvSI = red[RED_PI2U1];
+          b.U1pi2AvI = 0.5 * (b.U1pi2AvSIM + b.U1pi2AvSI);
+        }
+        st->K = sf_kernel(b, mc);
+        b.aDotSIM = b.aDotSI;
+        b.aDotSI += dt * st->K;
+        b.aDotI = (b.aDotSIM + b.aDotSI) / 2.0;
+        b.aIM = b.aI;
+        b.aI += dt * b.aDotSI;
+        b.aSI = (b.aIM + b.aI) / 2.0;
+        break;
+      case OP_LF_SYNC:  // leapfrog.h:105-112
+        if (mc.ns > 0) b.pi2AvI = red[RED_PI2S];
+        if (mc.ncs > 0) b.CSpi2AvI = red[RED_PI2CS];
+        if (mc.nu1 > 0) b.U1pi2AvI = red[RED_PI2U1];
+        st->K = sf_kernel(b, mc);
+        b.aDotI = b.aDotSI + dt / 2.0 * st->K;
+        break;
+      case OP_LF_INIT:  // leapfrog.h:35
+        st->K = sf_kernel(b, mc);
+        b.aDotSI = b.aDotI - dt / 2.0 * st->K;
+        break;
+      case OP_VV_SF_HALF:  // velocityverlet.h:165-168
+        st->K = sf_kernel(b, mc);
+        b.aDotSI = b.aDotI + (dt / 2) * st->K * w;
+        break;
+      case OP_VV_MOMENTA:  // velocityverlet.h:249-256
+        if (mc.ns > 0) b.pi2AvI = red[RED_PI2S];
+        if (mc.ncs > 0) b.CSpi2AvI = red[RED_PI2CS];
+        if (mc.nu1 > 0) b.U1pi2AvI = red[RED_PI2U1];
+        break;
+      case OP_VV_DRIFT_SF:  // velocityverlet.h:239-242
+        b.aIM = b.aI;
+        b.aI += dt * b.aDotSI * w;
+        b.aSI = (b.aIM + b.aI) / 2.0;
+        break;
+      case OP_VV_SF_ONE:  // velocityverlet.h:171-174
+        st->K = sf_kernel(b, mc);
+        b.aDotI = b.aDotSI + (dt / 2) * st->K * w;
+        break;
+      case OP_SET_ALL:  // averages.h:86-116
+        b.potAvI = red[RED_POT];
+        b.potAvSI = b.potAvI;
+        if (mc.ns > 0) {
+          b.pi2AvI = red[RED_PI2S];
+          b.pi2AvSI = b.pi2AvI;
+          b.grad2AvI = red[RED_GRAD2S];
+        }
+        if (mc.nu1 > 0) {
+          b.U1pi2AvI = red[RED_PI2U1];
+          b.U1pi2AvSI = b.U1pi2AvI;
+          b.U1Mag2AvI = red[RED_B2U1];
+        }
+        if (mc.ncs > 0) {
+          b.CSpi2AvI = red[RED_PI2CS];
+          b.CSpi2AvSI = b.CSpi2AvI;
+          b.CSgrad2AvI = red[RED_GRAD2CS];
+        }
+        break;
+      case OP_KERNEL_ONLY:
+        st->K = sf_kernel(b, mc);
+        break;
+      default:
+        break;
+    }
+  }
+  st->aK = pow(b.aI, 1 + mc.alpha);
+  st->aV = pow(b.aI, 3 + mc.alpha);
+  st->aG = pow(b.aI, mc.alpha - 1);
+  st->aJ = pow(b.aI, 1 + mc.alpha);
+  st->fdrift_s = pow(b.aSI, mc.alpha - 3);
+  st->fdrift_u = pow(b.aSI, mc.alpha - 1);
+}
+
+}  // namespace clmi
