@@ -24,6 +24,8 @@ HIP_SOURCES = [
     "scalar_pass.hip",
     "scalar_tiled.hip",
     "scalar_rows.hip",
+    "scalar_rows_f32.hip",
+    "scalar_rows_f64.hip",
     "gauge_pass.hip",
     "background.hip",
     "fft_kernels.hip",

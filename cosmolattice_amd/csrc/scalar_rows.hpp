@@ -1,0 +1,467 @@
+// Full-row scalar-singlet lattice pass (kernel template; instantiated per precision in
+// scalar_rows_f32.hip / scalar_rows_f64.hip) for N = 64, 128, 192, 256: same per-site arithmetic as
+// scalar_pass.hip (bit-identical fields), laid out so that no load is partial.
+//
+// A workgroup owns TY complete z-rows (WPR wavefronts per row, each lane VZ consecutive sites, read
+// and written with one vector access per array) and marches along x over XC planes.  x-neighbours
+// are kept in registers (prev / cur / next plane), y-neighbours come from an LDS tile holding the
+// TY rows of the current plane plus one halo row above and below, and z-neighbours are the lane's
+// own registers except at its two edges, which are read from the same LDS row.  Because rows are
+// complete there is no z-halo; the y-halo is two full rows per plane, the x-halo two planes per
+// chunk.  Outputs go out with non-temporal stores when NT.
+//
+// Volume sums are written per (group of G planes, TY-row y-tile), independent of XC.
+#pragma once
+
+#include "clmi_internal.hpp"
+#include "update_device.hpp"
+
+namespace clmi {
+namespace {
+
+__device__ __forceinline__ double ipow_d(double v, int e)
+{
+  double r = 1.0;
+  for (int i = 0; i < e; ++i) r *= v;
+  return r;
+}
+__device__ __forceinline__ float ipow_f(float v, int e)
+{
+  float r = 1.0f;
+  for (int i = 0; i < e; ++i) r *= v;
+  return r;
+}
+template <int NF>
+__device__ __forceinline__ double dpot_d(const ScalarPoly &P, const double *phi, int n)
+{
+  double d = 0.0;
+  for (int t = 0; t < P.nterms; ++t) {
+    const int e = P.e[t][n];
+    if (e == 0) continue;
+    double p = P.c[t] * e;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], f == n ? e - 1 : P.e[t][f]);
+    d += p;
+  }
+  return d;
+}
+template <int NF>
+__device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, int n)
+{
+  float d = 0.0f;
+  for (int t = 0; t < P.nterms; ++t) {
+    const int e = P.e[t][n];
+    if (e == 0) continue;
+    float p = (float)P.c[t] * (float)e;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], f == n ? e - 1 : P.e[t][f]);
+    d += p;
+  }
+  return d;
+}
+template <int NF>
+__device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
+{
+  double v = 0.0;
+  for (int t = 0; t < P.nterms; ++t) {
+    double p = P.c[t];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], P.e[t][f]);
+    v += p;
+  }
+  return v;
+}
+__device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
+{
+  const float s = lo + d;
+  const float h = hi + s;
+  const float bb = h - hi;
+  const float e = (hi - (h - bb)) + (s - bb);
+  hi = h;
+  lo = e;
+}
+
+// VZ consecutive elements with vector accesses (the site index of element 0 is a multiple of VZ).
+template <int VZ, typename S>
+__device__ __forceinline__ void vload(const S *p, uint32_t s, S (&o)[VZ])
+{
+  if constexpr (VZ == 4 && sizeof(S) == 4) {
+    typedef S v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const v4 *)(p + s);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else if constexpr (VZ % 2 == 0) {
+    typedef S v2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int k = 0; k < VZ; k += 2) {
+      const v2 v = *(const v2 *)(p + s + k);
+      o[k] = v.x;
+      o[k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) o[k] = p[s + k];
+  }
+}
+template <int VZ, bool NT, typename S>
+__device__ __forceinline__ void vstore(S *p, uint32_t s, const S (&o)[VZ])
+{
+  if constexpr (VZ == 4 && sizeof(S) == 4) {
+    typedef S v4 __attribute__((ext_vector_type(4)));
+    v4 v;
+    v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
+    if constexpr (NT) __builtin_nontemporal_store(v, (v4 *)(p + s)); else *(v4 *)(p + s) = v;
+  } else if constexpr (VZ % 2 == 0) {
+    typedef S v2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int k = 0; k < VZ; k += 2) {
+      v2 v;
+      v.x = o[k];
+      v.y = o[k + 1];
+      if constexpr (NT) __builtin_nontemporal_store(v, (v2 *)(p + s + k)); else *(v2 *)(p + s + k) = v;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+      if constexpr (NT) __builtin_nontemporal_store(o[k], p + s + k); else p[s + k] = o[k];
+    }
+  }
+}
+
+// Drifted field value and raw inputs per storage mode.
+template <bool F32>
+struct Tr;
+template <>
+struct Tr<false> {
+  using V = double;
+  using P = double;
+  template <int VZ>
+  struct Raw {
+    double f[VZ], p[VZ];
+  };
+  template <int VZ>
+  __device__ static __forceinline__ void load(const ScalarPassArgs &a, int f, uint32_t s, bool need_p, Raw<VZ> &r)
+  {
+    vload<VZ>((const double *)a.phi_in[f], s, r.f);
+    if (need_p) vload<VZ>((const double *)a.pi_in[f], s, r.p);
+    else
+#pragma unroll
+      for (int k = 0; k < VZ; ++k) r.p[k] = 0.0;
+  }
+  template <bool DRIFT, int VZ>
+  __device__ static __forceinline__ V value(const Raw<VZ> &r, int k, double fs, double cdt, float)
+  {
+    double v = r.f[k];
+    if (DRIFT) v = __dadd_rn(v, __dmul_rn(fs, __dmul_rn(cdt, r.p[k])));
+    return v;
+  }
+};
+template <>
+struct Tr<true> {
+  using V = float2;
+  using P = float;
+  template <int VZ>
+  struct Raw {
+    float h[VZ], l[VZ], p[VZ];
+  };
+  template <int VZ>
+  __device__ static __forceinline__ void load(const ScalarPassArgs &a, int f, uint32_t s, bool need_p, Raw<VZ> &r)
+  {
+    vload<VZ>((const float *)a.phi_in[f], s, r.h);
+    vload<VZ>(a.philo_in[f], s, r.l);
+    if (need_p) vload<VZ>((const float *)a.pi_in[f], s, r.p);
+    else
+#pragma unroll
+      for (int k = 0; k < VZ; ++k) r.p[k] = 0.0f;
+  }
+  template <bool DRIFT, int VZ>
+  __device__ static __forceinline__ V value(const Raw<VZ> &r, int k, double, double, float cf)
+  {
+    float h = r.h[k], l = r.l[k];
+    if (DRIFT) pair_add(h, l, cf * r.p[k]);
+    return make_float2(h, l);
+  }
+};
+
+template <int NW, int K>
+__device__ __forceinline__ void block_reduce_rows(double (&v)[K], double *row)
+{
+  __shared__ double sh[NW][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += sh[w][threadIdx.x];
+    row[threadIdx.x] = s;
+  }
+}
+
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, bool NT>
+__global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
+{
+  using T = Tr<F32>;
+  using V = typename T::V;
+  using Raw = typename T::template Raw<VZ>;
+  constexpr int NW = WPR * TY;  // wavefronts per workgroup
+  constexpr int N = 64 * VZ * WPR;
+  __shared__ V ldsbuf[2][NF][TY + 2][N];  // tile of plane x in buffer (x - x0) & 1
+
+  const uint32_t NN = (uint32_t)N * N;
+  const int nTy = N / TY;
+  const int nwg = gridDim.x, b = blockIdx.x;
+  const int L = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
+  const int ty = L % nTy, cx = L / nTy;
+  const int y0 = ty * TY, x0 = cx * XC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = wave / WPR;                    // 0..TY-1
+  const int zb = (wave % WPR) * 64 * VZ + lane * VZ;  // first z of this lane
+  const int y = y0 + row;
+
+  const double fs = DRIFT ? a.st->fdrift_s : 0.0;
+  const double aK = KICK ? a.st->aK : 0.0;
+  const double aV = KICK ? a.st->aV : 0.0;
+  const float cf = DRIFT ? (float)(a.st->fdrift_s * a.cdt) : 0.0f;
+  const float kK = KICK ? (float)(a.ckick * a.st->aK * a.inv_dx2) : 0.0f;
+  const float kV = KICK ? (float)(a.ckick * a.st->aV) : 0.0f;
+
+  auto wrapx = [&](int x) -> uint32_t { return (uint32_t)(x < 0 ? x + N : (x >= N ? x - N : x)); };
+  auto rowbase = [&](int x, int yy) -> uint32_t {
+    yy = yy < 0 ? yy + N : (yy >= N ? yy - N : yy);
+    return wrapx(x) * NN + (uint32_t)yy * N;
+  };
+  // halo rows: the waves of the first tile row also load row y0-1, those of the last row y0+TY
+  const bool halo = (row == 0) || (row == TY - 1);
+  const int hy = row == 0 ? y0 - 1 : y0 + TY;
+  const int hl = row == 0 ? 0 : TY + 1;
+
+  V prev[NF][VZ], cur[NF][VZ], nxt[NF][VZ];
+  typename T::P pic[NF][VZ], pin[NF][VZ];
+  Raw pf[NF], hq[NF];
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+
+  {
+  auto &lds = ldsbuf[0];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    Raw q;
+    T::load(a, f, rowbase(x0 - 1, y) + zb, DRIFT, q);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) prev[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
+    T::load(a, f, rowbase(x0, y) + zb, true, q);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+      cur[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
+      pic[f][k] = q.p[k];
+      lds[f][row + 1][zb + k] = cur[f][k];
+    }
+    T::load(a, f, rowbase(x0 + 1, y) + zb, true, q);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+      nxt[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
+      pin[f][k] = q.p[k];
+    }
+    if (halo) {
+      T::load(a, f, rowbase(x0, hy) + zb, DRIFT, q);
+#pragma unroll
+      for (int k = 0; k < VZ; ++k) lds[f][hl][zb + k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
+    }
+  }
+  }
+  __syncthreads();
+
+  for (int i = 0; i < XC; ++i) {
+    const int x = x0 + i;
+    const bool more = i + 1 < XC;
+    auto &lds = ldsbuf[i & 1];
+    if (more) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T::load(a, f, rowbase(x + 2, y) + zb, true, pf[f]);
+        if (halo) T::load(a, f, rowbase(x + 1, hy) + zb, DRIFT, hq[f]);
+      }
+    }
+
+    const uint32_t s0 = rowbase(x, y) + zb;
+    // z-edge neighbours of this lane from the LDS copy of its own row
+    V zl[NF], zr[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      zl[f] = lds[f][row + 1][zb == 0 ? N - 1 : zb - 1];
+      zr[f] = lds[f][row + 1][zb + VZ == N ? 0 : zb + VZ];
+    }
+
+    double p2 = 0.0, g2 = 0.0, vsum = 0.0;
+    typename T::P pout[NF][VZ];
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+      double phid[NF];
+      float phif[NF];
+      double lapd[NF];
+      float lapf[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const V c = cur[f][k];
+        const V vyp = lds[f][row + 2][zb + k], vym = lds[f][row][zb + k];
+        const V vzp = k + 1 < VZ ? cur[f][k + 1 < VZ ? k + 1 : k] : zr[f];
+        const V vzm = k > 0 ? cur[f][k > 0 ? k - 1 : k] : zl[f];
+        const V vxp = nxt[f][k], vxm = prev[f][k];
+        if constexpr (!F32) {
+          phid[f] = c;
+          if (KICK) {
+            double l = 0.0;
+            l += vxp + vxm - 2 * c;
+            l += vyp + vym - 2 * c;
+            l += vzp + vzm - 2 * c;
+            lapd[f] = l * a.inv_dx2;
+          }
+          if (RED) {
+            const double gx = (vxp - c) * a.inv_dx, gy = (vyp - c) * a.inv_dx, gz = (vzp - c) * a.inv_dx;
+            g2 += gx * gx;
+            g2 += gy * gy;
+            g2 += gz * gz;
+          }
+        } else {
+          phif[f] = c.x + c.y;
+          phid[f] = (double)c.x + (double)c.y;
+          const float dxp = (vxp.x - c.x) + (vxp.y - c.y);
+          const float dyp = (vyp.x - c.x) + (vyp.y - c.y);
+          const float dzp = (vzp.x - c.x) + (vzp.y - c.y);
+          if (KICK) {
+            const float dxm = (vxm.x - c.x) + (vxm.y - c.y);
+            const float dym = (vym.x - c.x) + (vym.y - c.y);
+            const float dzm = (vzm.x - c.x) + (vzm.y - c.y);
+            lapf[f] = ((dxp + dxm) + (dyp + dym)) + (dzp + dzm);
+          }
+          if (RED) {
+            const double gx = (double)dxp * a.inv_dx, gy = (double)dyp * a.inv_dx, gz = (double)dzp * a.inv_dx;
+            g2 += gx * gx;
+            g2 += gy * gy;
+            g2 += gz * gz;
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if constexpr (!F32) {
+          double p = pic[f][k];
+          if (KICK) p += a.ckick * (aK * lapd[f] - aV * dpot_d<NF>(a.poly, phid, f));
+          pout[f][k] = p;
+          p2 += p * p;
+        } else {
+          float p = pic[f][k];
+          if (KICK) p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
+          pout[f][k] = p;
+          p2 += (double)p * (double)p;
+        }
+      }
+      if (RED) vsum += pot_d<NF>(a.poly, phid);
+    }
+    // outputs
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (KICK) vstore<VZ, NT>((typename T::P *)a.pi_out[f], s0, pout[f]);
+      if (DRIFT) {
+        if constexpr (!F32) {
+          vstore<VZ, NT>((double *)a.phi_out[f], s0, cur[f]);
+        } else {
+          float h[VZ], l[VZ];
+#pragma unroll
+          for (int k = 0; k < VZ; ++k) {
+            h[k] = cur[f][k].x;
+            l[k] = cur[f][k].y;
+          }
+          vstore<VZ, NT>((float *)a.phi_out[f], s0, h);
+          vstore<VZ, NT>(a.philo_out[f], s0, l);
+        }
+      }
+    }
+    if (RED) {
+      acc0 += p2;
+      acc1 += g2;
+      acc2 += vsum;
+    }
+    if (RED && (i + 1) % G == 0) {
+      double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
+      const size_t prow = (size_t)(x / G) * nTy + ty;
+      block_reduce_rows<NW, kNumRed>(v, a.partials + prow * kNumRed);
+      acc0 = acc1 = acc2 = 0.0;
+    }
+    if (more) {
+      // the other buffer was last read in the previous iteration, which every wavefront finished
+      // before the barrier that ended it
+      auto &ldn = ldsbuf[(i + 1) & 1];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) {
+          prev[f][k] = cur[f][k];
+          cur[f][k] = nxt[f][k];
+          pic[f][k] = pin[f][k];
+          nxt[f][k] = T::template value<DRIFT>(pf[f], k, fs, a.cdt, cf);
+          pin[f][k] = pf[f].p[k];
+          ldn[f][row + 1][zb + k] = cur[f][k];
+        }
+        if (halo)
+#pragma unroll
+          for (int k = 0; k < VZ; ++k) ldn[f][hl][zb + k] = T::template value<DRIFT>(hq[f], k, fs, a.cdt, cf);
+      }
+      __syncthreads();  // LDS tile of plane x+1 ready
+    }
+  }
+
+  if (RED && a.fold) {
+    // finish the background update in the last workgroup to complete
+    __shared__ unsigned s_last;
+    __shared__ double tot[kNumRed];
+    __threadfence();
+    __syncthreads();
+    DevState *st = const_cast<DevState *>(a.st);
+    if (threadIdx.x == 0) s_last = atomicAdd(&st->done_count, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      sum_partials<64 * NW>(a.partials, a.fold_rows, tot);
+      if (threadIdx.x < kNumRed) st->red[threadIdx.x] = tot[threadIdx.x] * a.mc.invV;
+      if (threadIdx.x == 0) {
+        double red[kNumRed];
+        for (int k = 0; k < kNumRed; ++k) red[k] = tot[k] * a.mc.invV;
+        apply_update(st, red, a.mc, a.ops);
+        st->done_count = 0;
+      }
+    }
+  }
+}
+
+template <int NF, bool F32, int VZ, int WPR, int TY, bool NT>
+hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
+                         hipStream_t s)
+{
+  const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
+  const dim3 blk(64 * WPR * TY);
+#define CLMI_SR(D, K, R)                                                                                       \
+  case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                             \
+    hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, NT>), dim3(blocks), blk, 0, s, a, XC, G); \
+    break;
+  switch (code) {
+    CLMI_SR(true, true, true)
+    CLMI_SR(true, true, false)
+    CLMI_SR(true, false, true)
+    CLMI_SR(true, false, false)
+    CLMI_SR(false, true, true)
+    CLMI_SR(false, true, false)
+    CLMI_SR(false, false, true)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CLMI_SR
+  return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace clmi

@@ -1,0 +1,28 @@
+// Instantiations of the full-row scalar pass for FP64 storage.
+#include "scalar_rows.hpp"
+
+namespace clmi {
+
+hipError_t launch_scalar_rows_f64(int nf, bool drift, bool kick, bool red, const ScalarPassArgs &a,
+                                  const RowsPlan &p, hipStream_t s)
+{
+  const bool f32 = false;
+#define CLMI_SHAPE(NF_, F32_, VZ_, WPR_, TY_)                                                          \
+  if (nf == NF_ && f32 == F32_ && p.VZ == VZ_ && p.WPR == WPR_ && p.TY == TY_)                         \
+    return p.NT ? dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s) \
+                : dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+  CLMI_SHAPE(2, false, 2, 2, 8)
+  CLMI_SHAPE(2, false, 4, 1, 8)
+  CLMI_SHAPE(2, false, 2, 1, 8)
+  CLMI_SHAPE(2, false, 1, 1, 8)
+  CLMI_SHAPE(2, false, 3, 1, 8)
+  CLMI_SHAPE(1, false, 4, 1, 8)
+  CLMI_SHAPE(1, false, 2, 2, 8)
+  CLMI_SHAPE(1, false, 2, 1, 8)
+  CLMI_SHAPE(1, false, 1, 1, 8)
+  CLMI_SHAPE(1, false, 3, 1, 8)
+#undef CLMI_SHAPE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace clmi
