@@ -194,26 +194,15 @@ ScalarPassArgs scalar_args(const clmi_ctx *c, int fi, int fo, int pi, int po, do
 }
 
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
-// With fold_ops non-null and a single-sector model on the full-row kernel, the pass also finishes
-// the background update with those operations in its last workgroup (*folded = true).
-int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows,
-             const UpdateOps *fold_ops = nullptr, bool *folded = nullptr)
+int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
 {
   const bool timed = c->prof && drift && kick;
   if (timed) CHK(prof_event(c));
   int r = 0;
-  if (folded) *folded = false;
   if (c->cfg.n_scalars > 0) {
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
-    ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
-    if (c->rowk && red && fold_ops && folded && c->cfg.n_cscalars == 0) {
-      a.fold = 1;
-      a.fold_rows = c->rplan.rows;
-      a.mc = c->mc;
-      a.ops = *fold_ops;
-      *folded = true;
-    }
+    const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
     if (c->rowk) {
       HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream));
       if (red) r += c->rplan.rows;
@@ -289,16 +278,13 @@ int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> 
   return CLMI_OK;
 }
 
-// A lattice pass followed by the background update `ops` (skipped when do_update is false):
-// folded into the pass where possible, otherwise a separate update launch.
+// A lattice pass followed by the background update `ops` (skipped when do_update is false).
 int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g,
                 bool do_update, std::initializer_list<std::pair<int, double>> ops)
 {
-  const UpdateOps u = make_ops(ops);
   int rows = 0;
-  bool folded = false;
-  CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows, do_update ? &u : nullptr, &folded));
-  if (do_update && !folded) HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
+  CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows));
+  if (do_update) HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, make_ops(ops), c->stream));
   return CLMI_OK;
 }
 

@@ -44,9 +44,7 @@ struct DevState {
   double fdrift_s;  // aSI^(alpha-3)  -- scalar / complex-scalar drift factor
   double fdrift_u;  // aSI^(alpha-1)  -- U(1) drift factor
   double red[kNumRed];  // last finished volume averages
-  unsigned done_count;  // workgroups finished in a pass that completes the update in its last workgroup
-  unsigned pad0;
-  double pad[2];
+  double pad[3];
 };
 
 // Polynomial potential of the scalar singlets in program units.
@@ -115,12 +113,6 @@ struct ScalarPassArgs {
   const DevState *st;
   double *partials;  // this pass's rows, kNumRed doubles each
   ScalarPoly poly;
-  // When fold != 0 the last workgroup to finish sums fold_rows partial rows and applies ops to the
-  // background (the work of the separate update kernel).
-  int fold;
-  int fold_rows;
-  ModelConsts mc;
-  UpdateOps ops;
 };
 
 struct GaugePassArgs {

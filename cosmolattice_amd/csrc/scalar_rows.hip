@@ -21,7 +21,7 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, bool alt, RowsPl
   else if (N == 64) { vz = 1; wpr = 1; ty = 8; }
   else if (N == 192) { vz = 3; wpr = 1; ty = 8; }
   else return false;
-  const int G = 4;
+  const int G = 8;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
   p->VZ = vz;
   p->WPR = wpr;

@@ -14,7 +14,6 @@
 #pragma once
 
 #include "clmi_internal.hpp"
-#include "update_device.hpp"
 
 namespace clmi {
 namespace {
@@ -412,28 +411,6 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
           for (int k = 0; k < VZ; ++k) ldn[f][hl][zb + k] = T::template value<DRIFT>(hq[f], k, fs, a.cdt, cf);
       }
       __syncthreads();  // LDS tile of plane x+1 ready
-    }
-  }
-
-  if (RED && a.fold) {
-    // finish the background update in the last workgroup to complete
-    __shared__ unsigned s_last;
-    __shared__ double tot[kNumRed];
-    __threadfence();
-    __syncthreads();
-    DevState *st = const_cast<DevState *>(a.st);
-    if (threadIdx.x == 0) s_last = atomicAdd(&st->done_count, 1u) == gridDim.x - 1 ? 1u : 0u;
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      sum_partials<64 * NW>(a.partials, a.fold_rows, tot);
-      if (threadIdx.x < kNumRed) st->red[threadIdx.x] = tot[threadIdx.x] * a.mc.invV;
-      if (threadIdx.x == 0) {
-        double red[kNumRed];
-        for (int k = 0; k < kNumRed; ++k) red[k] = tot[k] * a.mc.invV;
-        apply_update(st, red, a.mc, a.ops);
-        st->done_count = 0;
-      }
     }
   }
 }

@@ -1,5 +1,4 @@
-// Device-side background update shared by the stand-alone update kernel and by lattice passes
-// that finish the update in their last workgroup.
+// Device-side background update (used by the update kernel).
 //
 // sum_partials: fixed-order sum of the per-workgroup partial rows (bit-reproducible for a given
 // block size).  apply_update: one thread advances the scale factor exactly as the reference's host
@@ -37,12 +36,10 @@ __device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &
 
 // Fixed-order sum of `rows` partial rows by a block of NT threads: thread t takes rows t, t + NT,
 // ... (four loads in flight per slot), then wavefront shuffles, then the wavefront sums in order.
-// Loads are volatile so that rows written by other workgroups of the same launch are read from
-// L2.  Result in out[0..kNumRed) (shared memory); ends with a barrier.
+// Result in out[0..kNumRed) (shared memory); ends with a barrier.
 template <int NT>
-__device__ inline void sum_partials(const double *partials_, int rows, double *out)
+__device__ inline void sum_partials(const double *partials, int rows, double *out)
 {
-  const volatile double *partials = partials_;
   __shared__ double sh[NT / 64][kNumRed];
   const int t = threadIdx.x;
   double v[kNumRed];
