@@ -65,6 +65,18 @@ __device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, i
   return d;
 }
 template <int NF>
+__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
+{
+  float v = 0.0f;
+  for (int t = 0; t < P.nterms; ++t) {
+    float p = (float)P.c[t];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
+    v += p;
+  }
+  return v;
+}
+template <int NF>
 __device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
 {
   double v = 0.0;
@@ -190,8 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_pass_f32(const ScalarPassArgs
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < V; s += gridDim.x * kBlock) {
     const Nbr nb = neighbours(s, N, NN);
     float phic[NF], lap[NF];
-    double phid[NF];
-    double g2 = 0.0;
+    float g2f = 0.0f;  // per-site terms in FP32, accumulated in FP64
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const float *H = (const float *)a.phi_in[f];
@@ -209,7 +220,6 @@ __global__ __launch_bounds__(kBlock) void k_scalar_pass_f32(const ScalarPassArgs
         a.philo_out[f][s] = cl;
       }
       phic[f] = ch + cl;
-      phid[f] = (double)ch + (double)cl;
       float xph, xpl, yph, ypl, zph, zpl;
       val(nb.xp, xph, xpl);
       val(nb.yp, yph, ypl);
@@ -227,14 +237,9 @@ __global__ __launch_bounds__(kBlock) void k_scalar_pass_f32(const ScalarPassArgs
         const float dzm = (zmh - ch) + (zml - cl);
         lap[f] = ((dxp + dxm) + (dyp + dym)) + (dzp + dzm);
       }
-      if (RED) {
-        const double gx = (double)dxp * a.inv_dx, gy = (double)dyp * a.inv_dx, gz = (double)dzp * a.inv_dx;
-        g2 += gx * gx;
-        g2 += gy * gy;
-        g2 += gz * gz;
-      }
+      if (RED) g2f += (dxp * dxp + dyp * dyp) + dzp * dzp;
     }
-    double p2 = 0.0;
+    float p2f = 0.0f;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       float p = ((const float *)a.pi_in[f])[s];
@@ -242,12 +247,12 @@ __global__ __launch_bounds__(kBlock) void k_scalar_pass_f32(const ScalarPassArgs
         p += kK * lap[f] - kV * dpot_f<NF>(a.poly, phic, f);
         ((float *)a.pi_out[f])[s] = p;
       }
-      p2 += (double)p * (double)p;
+      p2f += p * p;
     }
     if (RED) {
-      acc[0] += p2;
-      acc[1] += g2;
-      acc[2] += pot_d<NF>(a.poly, phid);
+      acc[0] += (double)p2f;
+      acc[1] += (double)g2f * a.inv_dx2;
+      acc[2] += (double)pot_f<NF>(a.poly, phic);
     }
   }
   if (RED) {

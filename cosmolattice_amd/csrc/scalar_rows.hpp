@@ -59,6 +59,18 @@ __device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, i
   return d;
 }
 template <int NF>
+__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
+{
+  float v = 0.0f;
+  for (int t = 0; t < P.nterms; ++t) {
+    float p = (float)P.c[t];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
+    v += p;
+  }
+  return v;
+}
+template <int NF>
 __device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
 {
   double v = 0.0;
@@ -303,6 +315,7 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
       float phif[NF];
       double lapd[NF];
       float lapf[NF];
+      float g2f = 0.0f, p2f = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const V c = cur[f][k];
@@ -327,7 +340,6 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
           }
         } else {
           phif[f] = c.x + c.y;
-          phid[f] = (double)c.x + (double)c.y;
           const float dxp = (vxp.x - c.x) + (vxp.y - c.y);
           const float dyp = (vyp.x - c.x) + (vyp.y - c.y);
           const float dzp = (vzp.x - c.x) + (vzp.y - c.y);
@@ -337,12 +349,7 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
             const float dzm = (vzm.x - c.x) + (vzm.y - c.y);
             lapf[f] = ((dxp + dxm) + (dyp + dym)) + (dzp + dzm);
           }
-          if (RED) {
-            const double gx = (double)dxp * a.inv_dx, gy = (double)dyp * a.inv_dx, gz = (double)dzp * a.inv_dx;
-            g2 += gx * gx;
-            g2 += gy * gy;
-            g2 += gz * gz;
-          }
+          if (RED) g2f += (dxp * dxp + dyp * dyp) + dzp * dzp;
         }
       }
 #pragma unroll
@@ -356,10 +363,18 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
           float p = pic[f][k];
           if (KICK) p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
           pout[f][k] = p;
-          p2 += (double)p * (double)p;
+          p2f += p * p;
         }
       }
-      if (RED) vsum += pot_d<NF>(a.poly, phid);
+      if (RED) {
+        if constexpr (!F32) {
+          vsum += pot_d<NF>(a.poly, phid);
+        } else {
+          p2 += (double)p2f;
+          g2 += (double)g2f * a.inv_dx2;
+          vsum += (double)pot_f<NF>(a.poly, phif);
+        }
+      }
     }
     // outputs
 #pragma unroll

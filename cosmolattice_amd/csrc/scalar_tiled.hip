@@ -59,6 +59,18 @@ __device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, i
   return d;
 }
 template <int NF>
+__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
+{
+  float v = 0.0f;
+  for (int t = 0; t < P.nterms; ++t) {
+    float p = (float)P.c[t];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
+    v += p;
+  }
+  return v;
+}
+template <int NF>
 __device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
 {
   double v = 0.0;
@@ -242,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
       const int ly = wave * RY + r + 1;
       const uint32_t s = site(x, y0 + ly - 1, z);
       double g2 = 0.0;
+      float g2f = 0.0f, p2f = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
       double phid[NF];
       float phif[NF];
       double lapd[NF];
@@ -270,7 +283,6 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
           }
         } else {
           phif[f] = c.x + c.y;
-          phid[f] = (double)c.x + (double)c.y;
           if (DRIFT) {
             store_out<NT>((float *)a.phi_out[f], s, c.x);
             store_out<NT>(a.philo_out[f], s, c.y);
@@ -284,12 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
             const float dzm = (vzm.x - c.x) + (vzm.y - c.y);
             lapf[f] = ((dxp + dxm) + (dyp + dym)) + (dzp + dzm);
           }
-          if (RED) {
-            const double gx = (double)dxp * a.inv_dx, gy = (double)dyp * a.inv_dx, gz = (double)dzp * a.inv_dx;
-            g2 += gx * gx;
-            g2 += gy * gy;
-            g2 += gz * gz;
-          }
+          if (RED) g2f += (dxp * dxp + dyp * dyp) + dzp * dzp;
         }
       }
       double p2 = 0.0;
@@ -308,13 +315,19 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
             p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
             store_out<NT>((float *)a.pi_out[f], s, p);
           }
-          p2 += (double)p * (double)p;
+          p2f += p * p;
         }
       }
       if (RED) {
-        acc0 += p2;
-        acc1 += g2;
-        acc2 += pot_d<NF>(a.poly, phid);
+        if constexpr (!F32) {
+          acc0 += p2;
+          acc1 += g2;
+          acc2 += pot_d<NF>(a.poly, phid);
+        } else {
+          acc0 += (double)p2f;
+          acc1 += (double)g2f * a.inv_dx2;
+          acc2 += (double)pot_f<NF>(a.poly, phif);
+        }
       }
     }
 

@@ -9,6 +9,24 @@
 
 namespace clmi {
 
+// a^e: integer exponents (|e| <= 32, the common case alpha = 1) by binary powering, which is a
+// few multiplications instead of a serial log/exp evaluation on the update's single thread;
+// anything else by pow().  Agrees with pow() to a few ulp.
+__device__ inline double rpow(double a, double e)
+{
+  if (e == rint(e) && fabs(e) <= 32.0) {
+    int n = (int)fabs(e);
+    double p = 1.0, q = a;
+    while (n) {
+      if (n & 1) p *= q;
+      q *= q;
+      n >>= 1;
+    }
+    return e < 0 ? 1.0 / p : p;
+  }
+  return pow(a, e);
+}
+
 // ScaleFactorKernels::get (evolvers/kernels/scalefactorkernels.h:27-81) with the energy
 // normalisations of definitions/energies.h:37-85.
 __device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &mc)
@@ -16,20 +34,20 @@ __device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &
   const double a = b.aI;
   double Eks = 0, Egs = 0, Ekcs = 0, Egcs = 0, Eel = 0, Emag = 0;
   if (mc.ns > 0) {
-    Eks = 0.5 * b.pi2AvI * pow(a, -6.0);
-    Egs = 0.5 * b.grad2AvI * pow(a, -2.0);
+    Eks = 0.5 * b.pi2AvI * rpow(a, -6.0);
+    Egs = 0.5 * b.grad2AvI * rpow(a, -2.0);
   }
   if (mc.ncs > 0) {
-    Ekcs = b.CSpi2AvI * pow(a, -6.0);
-    Egcs = b.CSgrad2AvI * pow(a, -2.0);
+    Ekcs = b.CSpi2AvI * rpow(a, -6.0);
+    Egcs = b.CSgrad2AvI * rpow(a, -2.0);
   }
   if (mc.nu1 > 0) {
     const double w2 = (mc.omegaStar / mc.fStar) * (mc.omegaStar / mc.fStar);
-    Eel = 0.5 * w2 * pow(a, -4.0) * b.U1pi2AvI;
-    Emag = 0.5 * w2 * pow(a, -4.0) * b.U1Mag2AvI;
+    Eel = 0.5 * w2 * rpow(a, -4.0) * b.U1pi2AvI;
+    Emag = 0.5 * w2 * rpow(a, -4.0) * b.U1Mag2AvI;
   }
   const double fm = mc.fStar / mc.MPl;
-  return pow(a, 2 * mc.alpha + 1) / 3.0 * (fm * fm) *
+  return rpow(a, 2 * mc.alpha + 1) / 3.0 * (fm * fm) *
          ((mc.alpha - 2) * (Eks + Ekcs) + mc.alpha * (Egs + Egcs) + (mc.alpha - 1) * (Eel + Emag) +
           (mc.alpha + 1) * b.potAvI);
 }
@@ -168,12 +186,12 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
         break;
     }
   }
-  st->aK = pow(b.aI, 1 + mc.alpha);
-  st->aV = pow(b.aI, 3 + mc.alpha);
-  st->aG = pow(b.aI, mc.alpha - 1);
-  st->aJ = pow(b.aI, 1 + mc.alpha);
-  st->fdrift_s = pow(b.aSI, mc.alpha - 3);
-  st->fdrift_u = pow(b.aSI, mc.alpha - 1);
+  st->aK = rpow(b.aI, 1 + mc.alpha);
+  st->aV = rpow(b.aI, 3 + mc.alpha);
+  st->aG = rpow(b.aI, mc.alpha - 1);
+  st->aJ = rpow(b.aI, 1 + mc.alpha);
+  st->fdrift_s = rpow(b.aSI, mc.alpha - 3);
+  st->fdrift_u = rpow(b.aSI, mc.alpha - 1);
 }
 
 }  // namespace clmi
