@@ -157,8 +157,10 @@ def main():
         sec, kms, launches, bpl = timed_run(lat, args.warmup, args.steps)
         res[name] = {"seconds": sec, "kernel_ms": kms, "launches": launches, "bytes_per_launch": bpl,
                      "site_updates_per_s": N ** 3 * args.steps / sec, "ms_per_step": 1e3 * sec / args.steps,
+                     "kernel": next((ln[len("# kernel "):] for ln in tune.splitlines()
+                                     if ln.startswith("# kernel ")), "generic"),
                      "autotune_us": {ln.rsplit(" ", 1)[0]: float(ln.rsplit(" ", 1)[1])
-                                     for ln in tune.strip().splitlines()}}
+                                     for ln in tune.strip().splitlines() if not ln.startswith("#")}}
         log(f"[bench] {name}: {res[name]['ms_per_step']:.4f} ms/step, "
             f"{res[name]['site_updates_per_s'] / 1e9:.2f} G site-updates/s")
         lats[name] = lat
@@ -184,7 +186,7 @@ def main():
         achieved = r["bytes_per_launch"] / avg_s / 1e9
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_scalar_tiled (fused drift+kick+reductions)",
+                "kernel": r["kernel"] + " (fused drift+kick+reductions)",
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"],
                 "autotune_us": r["autotune_us"]}
 

@@ -974,10 +974,16 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CLMI_OK) return rc;
-  if (c->rowk)
+  if (c->rowk) {
     c->rplan = best_rplan;
-  else
+    rep = "# kernel k_scalar_rows VZ=" + std::to_string(best_rplan.VZ) + " WPR=" + std::to_string(best_rplan.WPR) +
+          " TY=" + std::to_string(best_rplan.TY) + " XC=" + std::to_string(best_rplan.XC) +
+          " NT=" + std::to_string((int)best_rplan.NT) + "\n" + rep;
+  } else {
     c->tplan = best_plan;
+    rep = "# kernel k_scalar_tiled XC=" + std::to_string(best_plan.XC) + " NT=" + std::to_string((int)best_plan.NT) +
+          "\n" + rep;
+  }
   if (report && report_len) {
     std::strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = '\0';

@@ -14,6 +14,14 @@
  * periodic boundaries.  Spatial direction i = 1, 2, 3 is x, y, z.
  *
  * Reference files cited relative to the CosmoLattice source tree.
+ *
+ * Pinning (tests/test_oracle.py): closed forms and invariants pin the Laplacian, the potentials,
+ * kick/drift (exact free-wave recurrence, time reversibility), the scale-factor update and the
+ * reductions (O(dt^2) Friedmann and energy constraints), the U(1) sector (gauge invariance, Gauss
+ * law, gQ = 0 limit) and the spectrum bins (Parseval, counts, single-mode bin).
+ * Parity unpinned against the reference (TempLat could not be built or imported): the Gauss-ratio
+ * denominator of orc_gauss_u1, which follows definitions/gausslaws.h:43-47 as written.  The
+ * spectrum prefactors and the GRF random stream live in oracle.py (same status, stated there).
  */
 #include <math.h>
 #include <stdint.h>

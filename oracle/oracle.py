@@ -11,6 +11,12 @@ Two parts:
   (measurements/powerspectrum.h:85-134) and the Gaussian-random-field initial conditions
   (initializers/fluctuationsgenerator.h:39-71,103-150) with the Philox4x32-10 generator.
 
+Pinning (tests/test_oracle.py): the bins by Parseval, mode counts and a single plane wave; the GRF
+by its shell statistics, Hermitian structure, cutoff and homogeneous values; Philox by the published
+known-answer vectors.  Parity unpinned against the reference (TempLat could not be built or
+imported): the random stream itself (TempLat's generator is replaced by Philox4x32-10 with a
+documented key/counter).
+
 Reference files are cited relative to the CosmoLattice source tree.
 """
 from __future__ import annotations
