@@ -84,5 +84,43 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
+HOST = PKG / "host"
+BIN = HOST / "_bin"
+# Model main programs (the reference builds one executable per model header).
+HOST_MODELS = {"lphi4": "models/lphi4.h", "lphi4_1f": "models/lphi4_1f.h", "lphi4U1": "models/lphi4U1.h"}
+
+
+def _gxx() -> str:
+    g = shutil.which("g++")
+    if not g:
+        raise RuntimeError("g++ not found; the host programs cannot be built")
+    return g
+
+
+def host_compile_cmd(src: Path, out: Path, model_header: str, model_class: str) -> list:
+    """g++ command for a C++ program over the CosmoInterface mirror and libclmi.so."""
+    return [_gxx(), "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{HOST}", f"-I{INCLUDE}",
+            f'-DCLMI_MODEL_HEADER="{model_header}"', f"-DCLMI_MODEL_CLASS={model_class}", str(src),
+            "-o", str(out), f"-L{PKG}", "-lclmi", f"-Wl,-rpath,{PKG}"]
+
+
+def build_host(verbose: bool = True) -> Path:
+    """Compile the model main programs (host/cosmolattice_main.cpp) into host/_bin/<model>."""
+    BIN.mkdir(exist_ok=True)
+    with _cf.ThreadPoolExecutor(max_workers=len(HOST_MODELS)) as ex:
+        futs = {}
+        for name, header in HOST_MODELS.items():
+            cmd = host_compile_cmd(HOST / "cosmolattice_main.cpp", BIN / name, header, name)
+            futs[name] = ex.submit(subprocess.run, cmd, capture_output=True, text=True)
+        for name, f in futs.items():
+            r = f.result()
+            if r.returncode != 0:
+                raise RuntimeError(f"host build failed for {name}:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {', '.join(str(BIN / n) for n in HOST_MODELS)}")
+    return BIN
+
+
 if __name__ == "__main__":
     build_library()
+    build_host()

@@ -1,0 +1,153 @@
+"""GPU: the C++ host mirror's main programs (cosmolattice_amd/host/_bin/<model>, built from
+host/cosmolattice_main.cpp) run the reference's time loop on the GPU and match the FP64 oracle.
+
+Each case writes the initial state as raw fields, runs the model program with a parameter file
+(measurement cadence, evolver), reads back its dump and its average_energy_conservation.txt, and
+replays the same loop on the oracle: sync + Friedmann check at every output step, then evolve.
+"""
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cosmolattice_amd.clmi import CLMI_F64, FLD_S, PI_S, Lattice, VV_WEIGHTS
+from test_gpu_parity import bench_spec, say, scalar_ics
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "cosmolattice_amd" / "host" / "_bin"
+FP64_TOL = 1e-10
+BG_NAMES = ["aI", "aSI", "aIM", "aDotI", "aDotSI", "aDotSIM", "pi2AvI", "pi2AvSI", "pi2AvSIM", "grad2AvI",
+            "CSpi2AvI", "CSpi2AvSI", "CSpi2AvSIM", "CSgrad2AvI", "U1pi2AvI", "U1pi2AvSI", "U1pi2AvSIM",
+            "U1Mag2AvI", "potAvI", "potAvSI"]
+
+
+def lphi4_params(N, dx, dt, steps, out_every, evolver="LF", extra="", geometry=None):
+    # lambda = 1, phi0 = 5.6964e18 GeV (fStar/MPl = 2.34), pi0 = -0.5 in program units
+    phi0 = 5.6964e18
+    geometry = geometry or f"lSide = {N * dx!r}"
+    return (f"N = {N}\n{geometry}\ndt = {dt!r}\ntMax = {steps * dt!r}\n"
+            f"tOutputFreq = {out_every * dt!r}\ntOutputInfreq = {out_every * dt!r}\n"
+            f"tOutputRareFreq = {out_every * dt!r}\nevolver = {evolver}\nexpansion = true\n"
+            f"print_headers = true\nlambda = 1\nq = 100\ninitial_amplitudes = {phi0!r} 0\n"
+            f"initial_momenta = {-0.5 * phi0 * phi0!r} 0\n{extra}")
+
+
+def write_state(d: Path, arrays: dict, bg: dict):
+    d.mkdir(parents=True, exist_ok=True)
+    for (k, i, c), a in arrays.items():
+        np.ascontiguousarray(a, dtype="<f8").tofile(d / f"{k}_{i}_{c}.f64")
+    (d / "background.txt").write_text("\n".join(repr(float(bg.get(n, 0.0))) for n in BG_NAMES) + "\n")
+
+
+def read_state(d: Path, keys, N):
+    arrays = {key: np.fromfile(d / f"{key[0]}_{key[1]}_{key[2]}.f64", dtype="<f8").reshape(N, N, N) for key in keys}
+    vals = [float(x) for x in (d / "background.txt").read_text().split()]
+    return arrays, dict(zip(BG_NAMES, vals))
+
+
+def run_program(model, tmp_path, params, *args):
+    exe = BIN / model
+    if not exe.exists():
+        pytest.fail(f"{exe} is not built (run the build first)")
+    pf = tmp_path / "run.in"
+    pf.write_text(params)
+    r = subprocess.run([str(exe), f"input={pf}", f"outputfile={tmp_path}/", *args], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def oracle_loop(orc, steps, out_every, evolver="LF"):
+    recs = []
+    if evolver == "LF":
+        orc.lf_init()
+    for i in range(steps):
+        if i % out_every == 0:
+            if evolver == "LF":
+                orc.lf_sync()
+            recs.append(orc.hubble_constraint())
+        if evolver == "LF":
+            orc.lf_evolve(1)
+        else:
+            orc.vv_evolve(VV_WEIGHTS[evolver], 1)
+    return np.array(recs)
+
+
+@pytest.mark.parametrize("evolver", ["LF", "VV4"])
+def test_host_program_matches_oracle(gpu, oracle_mod, tmp_path, evolver):
+    say(f"case host lphi4 {evolver}")
+    spec = bench_spec()
+    N, dx, dt, steps, every = 16, 0.4909, 0.005, 60, 10
+    ics = scalar_ics(spec, N, np.random.default_rng(7))
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=True)
+    for (k, i), a in ics.items():
+        orc.upload(k, i, 0, a)
+    orc.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=1.0)
+    _, _, rhs = orc.hubble_constraint()
+    orc.set_background(aDotI=math.sqrt(rhs))
+    write_state(tmp_path / "in", {(k, i, 0): a for (k, i), a in ics.items()}, orc.background())
+    run_program("lphi4", tmp_path, lphi4_params(N, dx, dt, steps, every, evolver),
+                f"clmi_load_dir={tmp_path / 'in'}", f"clmi_dump_dir={tmp_path}")
+    recs = oracle_loop(orc, steps, every, evolver)
+    keys = [(k, i, 0) for (k, i) in ics]
+    got, bg = read_state(tmp_path, keys, N)
+    for (k, i, c) in keys:
+        ref = orc.download(k, i)
+        e = float(np.max(np.abs(got[(k, i, c)] - ref)) / np.max(np.abs(ref)))
+        say((k, i), e)
+        assert e < FP64_TOL
+    bo = orc.background()
+    for n in BG_NAMES:
+        if bo[n] != 0:
+            assert bg[n] == pytest.approx(bo[n], rel=FP64_TOL), n
+    cons = np.loadtxt(tmp_path / "average_energy_conservation.txt", ndmin=2)
+    assert cons.shape == (len(recs), 4)
+    np.testing.assert_allclose(cons[:, 0], np.arange(len(recs)) * every * dt, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(cons[:, 2:], recs[:, 1:], rtol=FP64_TOL)
+    np.testing.assert_allclose(cons[:, 1], recs[:, 0], rtol=1e-6, atol=1e-13)
+    assert (tmp_path / "average_energy_conservation.txt").read_text().startswith(
+        "#t    rel_diff_friedmann    LHS_friedmann    RHS_friedmann\n")
+
+
+def fnv1a64(s: str) -> int:
+    h = 1469598103934665603
+    for ch in s.encode():
+        h ^= ch
+        h = (h * 1099511628211) % (1 << 64)
+    return h
+
+
+def test_host_initializer_matches_python_path(gpu, oracle_mod, tmp_path):
+    """ModelInitializer through the model program equals the same steps through the Python binding
+    (scale factor from the homogeneous energy, GRF per field, aDot from the Friedmann constraint),
+    and its fields equal the numpy GRF restatement."""
+    say("case host initializer")
+    spec = bench_spec()
+    N, kIR, dt = 32, 0.4, 0.005
+    dx = 2 * math.pi / kIR / N
+    kcut = N * kIR / 4
+    run_program("lphi4", tmp_path, lphi4_params(N, dx, dt, 0, 1, extra=f"kCutOff = {kcut!r}\nbaseSeed = 42\n",
+                                                geometry=f"kIR = {kIR!r}"),
+                f"clmi_dump_dir={tmp_path}")
+    keys = [(FLD_S, 0, 0), (PI_S, 0, 0), (FLD_S, 1, 0), (PI_S, 1, 0)]
+    got, bg = read_state(tmp_path, keys, N)
+    lat = Lattice(spec, N, dx, dt, CLMI_F64, expansion=True)
+    aDot0 = spec.fStar / spec.MPl * math.sqrt((0.5 * spec.pi0[0] ** 2 + spec.potential(spec.fld0)) / 3.0)
+    lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=aDot0)
+    m2 = spec.mass2(spec.fld0)
+    seed = fnv1a64("42")
+    for f in range(2):
+        lat.grf_scalar(f, seed, f, m2[f], aDot0, kIR, kcut, spec.fld0[f], spec.pi0[f])
+        phi_o, pi_o = oracle_mod.grf_scalar(N, dx, spec, seed, f, m2[f], aDot0, kIR, kcut, spec.fld0[f], spec.pi0[f])
+        # kIR and dx pass through the run parameters' geometry rule, so allow last-bit differences
+        for k, ref in ((FLD_S, lat.download(FLD_S, f, 0)), (PI_S, lat.download(PI_S, f, 0))):
+            assert np.max(np.abs(got[(k, f, 0)] - ref)) <= 1e-13 * np.max(np.abs(ref))
+        assert np.max(np.abs(got[(FLD_S, f, 0)] - phi_o)) < 1e-12 * np.max(np.abs(phi_o - spec.fld0[f]))
+    _, _, rhs = lat.hubble_constraint()
+    assert bg["aDotI"] == pytest.approx(math.sqrt(rhs), rel=1e-14)
+    assert bg["aI"] == 1.0
+    lat.close()
