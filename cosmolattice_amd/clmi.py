@@ -113,6 +113,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "clmi_grf_scalar": (C.c_int, [P, C.c_int, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double]),
+        "clmi_grf_u1": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                  C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "clmi_profile_enable": (C.c_int, [P, C.c_int]),
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
@@ -163,6 +165,19 @@ class ModelSpec:
                 p *= phi[f] ** ef
             v += p
         return v
+
+    def cs_mass2(self, comps: Sequence[float]) -> list:
+        """Effective masses^2 of the two complex-scalar components at the homogeneous point
+        (Potential::deriv2CS, definitions/potential.h:36-43,83-88):
+        m_c^2 = V''(|phi|) phi_c^2/|phi|^2 + V'(|phi|)/|phi| (1 - phi_c^2/|phi|^2), with the
+        reference's 1e-14 floor on |phi| (abstractmodel.h:255-274)."""
+        r = math.sqrt(comps[0] ** 2 + comps[1] ** 2)
+        if r == 0.0:
+            comps = [1e-14 * 0.70710678118654752, 1e-14 * 0.70710678118654752]
+            r = math.sqrt(comps[0] ** 2 + comps[1] ** 2)
+        d1 = sum(c * p * r ** (p - 1) for c, p in self.cs_terms if p >= 1)
+        d2 = sum(c * p * (p - 1) * r ** (p - 2) for c, p in self.cs_terms if p >= 2)
+        return [d2 * x * x / (r * r) + d1 / r * (1 - x * x / (r * r)) for x in comps]
 
     def mass2(self, phi: Sequence[float]) -> list:
         """Diagonal second derivatives of V at phi (the reference's potDeriv2 at the homogeneous
@@ -380,6 +395,14 @@ class Lattice:
                    kCutOff: float, fld0: float, pi0: float):
         self._check(self.lib.clmi_grf_scalar(self._h, index, seed, stream, mass2, aDot, kIR, kCutOff, fld0, pi0),
                     "grf_scalar")
+
+    def grf_u1(self, seed: int, stream: int, mass2: Sequence[float], aDot: float, kIR: float, kCutOff: float,
+               fld0: Sequence[float], pi0: Sequence[float]):
+        """Complex scalar + U(1) vacuum initial conditions with the Gauss constraint (clmi_grf_u1)."""
+        m = (C.c_double * 2)(*mass2)
+        f = (C.c_double * 2)(*fld0)
+        p = (C.c_double * 2)(*pi0)
+        self._check(self.lib.clmi_grf_u1(self._h, seed, stream, m, aDot, kIR, kCutOff, f, p), "grf_u1")
 
     # -- timing ------------------------------------------------------------------------------------
     def profile_enable(self, on: bool = True):

@@ -864,6 +864,20 @@ int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, in
   return CLMI_OK;
 }
 
+namespace {
+// largest |n|^2 with sqrt(|n|^2) * kIR <= kCutOff (-1: no mode)
+int64_t cut_n2max(int N, double kIR, double kCutOff)
+{
+  const double r = kCutOff / kIR;
+  if (!(r >= 0)) return -1;
+  if (r > 4.0 * N) return (int64_t)3 * N * N;
+  int64_t n2max = (int64_t)std::floor(r * r);
+  while (std::sqrt((double)(n2max + 1)) * kIR <= kCutOff) ++n2max;
+  while (n2max >= 0 && std::sqrt((double)n2max) * kIR > kCutOff) --n2max;
+  return n2max;
+}
+}  // namespace
+
 int clmi_grf_scalar(clmi_ctx *c, int index, uint64_t seed, uint32_t stream, double mass2, double aDot, double kIR,
                     double kCutOff, double fld0, double pi0)
 {
@@ -887,19 +901,7 @@ int clmi_grf_scalar(clmi_ctx *c, int index, uint64_t seed, uint32_t stream, doub
   a.sigma_pref = c->cfg.omegaStar / c->cfg.fStar * std::pow(L / (dx * dx), 1.5) / std::sqrt(2.0);
   a.phik = c->fk;
   a.pik = c->fk2;
-  // largest |n|^2 with sqrt(|n|^2) * kIR <= kCutOff
-  int64_t n2max;
-  const double r = kCutOff / kIR;
-  if (!(r >= 0))
-    n2max = -1;
-  else if (r > 4.0 * N)
-    n2max = (int64_t)3 * N * N;
-  else {
-    n2max = (int64_t)std::floor(r * r);
-    while (std::sqrt((double)(n2max + 1)) * kIR <= kCutOff) ++n2max;
-    while (n2max >= 0 && std::sqrt((double)n2max) * kIR > kCutOff) --n2max;
-  }
-  HIPCHK(c, launch_grf_modes(a, n2max, c->stream));
+  HIPCHK(c, launch_grf_modes(a, cut_n2max(N, kIR, kCutOff), c->stream));
   const double scale = 1.0 / ((double)N * N * N);
   DevArr &f = c->fS[index][c->cur_fS];
   DevArr &p = c->pS[index][c->cur_pS];
@@ -907,6 +909,66 @@ int clmi_grf_scalar(clmi_ctx *c, int index, uint64_t seed, uint32_t stream, doub
   HIPCHK(c, launch_grf_finish(c->stage, scale, fld0, f.p, f.lo, c->f32, c->V, c->stream));
   FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk2, (hipfftDoubleReal *)c->stage));
   HIPCHK(c, launch_grf_finish(c->stage, scale, pi0, p.p, nullptr, c->f32, c->V, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+int clmi_grf_u1(clmi_ctx *c, uint64_t seed, uint32_t stream, const double mass2[2], double aDot, double kIR,
+                double kCutOff, const double fld0[2], const double pi0[2])
+{
+  CHK(check_ctx(c));
+  if (c->cfg.n_cscalars != 1) return fail(c, CLMI_E_INVALID, "the model has no complex scalar");
+  if (!mass2 || !fld0 || !pi0) return fail(c, CLMI_E_INVALID, "null argument");
+  if (!(kIR > 0)) return fail(c, CLMI_E_INVALID, "kIR must be > 0");
+  CHK(flush(c));
+  CHK(ensure_fft(c, c->cfg.n_u1 > 0, true, true));
+  const int N = c->N;
+  const double dx = c->cfg.dx;
+  const double L = N * dx;
+  const double scale = 1.0 / ((double)N * N * N);
+  GrfCsArgs a{};
+  a.N = N;
+  a.nmodes = (uint32_t)N * N * (N / 2 + 1);
+  a.seed = seed;
+  a.stream = stream;
+  a.mass2[0] = mass2[0];
+  a.mass2[1] = mass2[1];
+  a.aDot = aDot;
+  a.kIR = kIR;
+  // the norm of fluctuationsgenerator.h:39-71 (both components share it; their masses differ)
+  a.sigma_pref = c->cfg.omegaStar / c->cfg.fStar * std::pow(L / (dx * dx), 1.5) / std::sqrt(2.0);
+  a.phik = c->fk;
+  a.pik = c->fk2;
+  const int64_t n2max = cut_n2max(N, kIR, kCutOff);
+  for (int comp = 0; comp < 2; ++comp) {
+    a.comp = comp;
+    HIPCHK(c, launch_grf_cs_modes(a, n2max, c->stream));
+    DevArr &f = c->fCS[comp][c->cur_fG];
+    DevArr &p = c->pCS[comp][c->cur_pG];
+    FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk, (hipfftDoubleReal *)c->stage));
+    HIPCHK(c, launch_grf_finish(c->stage, scale, fld0[comp], f.p, f.lo, c->f32, c->V, c->stream));
+    FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk2, (hipfftDoubleReal *)c->stage));
+    HIPCHK(c, launch_grf_finish(c->stage, scale, pi0[comp], p.p, nullptr, c->f32, c->V, c->stream));
+  }
+  if (c->cfg.n_u1 > 0) {
+    // A = 0; pi_A from the Gauss constraint sourced by the complex scalar's charge
+    const DevArr &f0 = c->fCS[0][c->cur_fG], &f1 = c->fCS[1][c->cur_fG];
+    const DevArr &p0 = c->pCS[0][c->cur_pG], &p1 = c->pCS[1][c->cur_pG];
+    const double fw = c->cfg.fStar / c->cfg.omegaStar;
+    const double coef = -dx * dx * (2.0 / dx) * fw * fw * c->cfg.gQ;
+    HIPCHK(c, launch_u1_source(f0.p, f0.lo, f1.p, f1.lo, p0.p, p1.p, c->f32, coef, c->stage, c->V, c->stream));
+    FFTCHK(c, hipfftExecD2Z(c->plan_r2c, (hipfftDoubleReal *)c->stage, (hipfftDoubleComplex *)c->fk));
+    for (int d = 0; d < 3; ++d) {
+      DevArr &A = c->fU[d][c->cur_fG];
+      DevArr &P = c->pU[d][c->cur_pG];
+      const size_t eb = c->f32 ? sizeof(float) : sizeof(double);
+      HIPCHK(c, hipMemsetAsync(A.p, 0, eb * c->V, c->stream));
+      if (A.lo) HIPCHK(c, hipMemsetAsync(A.lo, 0, sizeof(float) * c->V, c->stream));
+      HIPCHK(c, launch_gauss_solve(c->fk, c->fk2, d, N, c->stream));
+      FFTCHK(c, hipfftExecZ2D(c->plan_c2r, (hipfftDoubleComplex *)c->fk2, (hipfftDoubleReal *)c->stage));
+      HIPCHK(c, launch_grf_finish(c->stage, scale, 0.0, P.p, nullptr, c->f32, c->V, c->stream));
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLMI_OK;
 }

@@ -248,6 +248,22 @@ struct GrfArgs {
   void *pik;   // double2
 };
 hipError_t launch_grf_modes(const GrfArgs &a, int64_t n2max, hipStream_t s);
+struct GrfCsArgs {
+  int N;
+  uint32_t nmodes;
+  uint64_t seed;
+  uint32_t stream;
+  int comp;          // complex-scalar component 0 or 1
+  double mass2[2];   // effective masses^2 of the two components
+  double aDot, kIR;
+  double sigma_pref;
+  void *phik;
+  void *pik;
+};
+hipError_t launch_grf_cs_modes(const GrfCsArgs &a, int64_t n2max, hipStream_t s);
+hipError_t launch_u1_source(const void *f0, const float *f0lo, const void *f1, const float *f1lo, const void *p0,
+                            const void *p1, int f32, double coef, double *out, uint32_t V, hipStream_t s);
+hipError_t launch_gauss_solve(const void *j0, void *out, int dir, int N, hipStream_t s);
 hipError_t launch_grf_finish(const double *src, double scale, double add, void *dst, float *dstlo, int f32,
                              uint32_t V, hipStream_t s);
 

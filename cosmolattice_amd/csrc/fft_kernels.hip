@@ -208,6 +208,140 @@ __global__ __launch_bounds__(256) void k_grf_modes(const GrfArgs a, int64_t n2ma
   }
 }
 
+// Complex-scalar fluctuations from constrained phases (u1initializer.h:157-230) for component
+// `comp` (0 or 1): per mode, Rayleigh amplitudes a_c and unit phases e00, e01, e10 drawn from
+// Philox (stream, counter (mode, draw 0..2)); e11 = e01 e10 conj(e00) makes every mode's
+// contribution to the total charge vanish.
+//   phi_c = a_c (e_c0 + e_c1) / 2,  pi_c = -i omega_0 a_c (e_c0 - e_c1) / 2 - aDot phi_c.
+// On the kz = 0 and kz = N/2 planes the mode -n takes the draws of n with
+// (e_c0, e_c1) -> (conj e_c1, conj e_c0), which keeps the fields real and the phase constraint;
+// self-conjugate modes use e01 = conj e00 and e10 = e00.
+__global__ __launch_bounds__(256) void k_grf_cs_modes(const GrfCsArgs a, int64_t n2max)
+{
+  const uint32_t N = a.N, Nh = N / 2 + 1;
+  for (uint32_t m = blockIdx.x * 256 + threadIdx.x; m < a.nmodes; m += gridDim.x * 256) {
+    const uint32_t kz = m % Nh;
+    const uint32_t kxy = m / Nh;
+    const uint32_t kx = kxy / N, ky = kxy - kx * N;
+    const int nx = kx <= N / 2 ? (int)kx : (int)kx - (int)N;
+    const int ny = ky <= N / 2 ? (int)ky : (int)ky - (int)N;
+    const int64_t n2 = (int64_t)nx * nx + (int64_t)ny * ny + (int64_t)kz * kz;
+    double2 ph = make_double2(0.0, 0.0), pp = make_double2(0.0, 0.0);
+    if (m != 0 && n2 <= n2max) {
+      const double k = sqrt((double)n2) * a.kIR;
+      const double m2c = a.mass2[a.comp] > 0 ? a.mass2[a.comp] : 0.0;
+      const double m20 = a.mass2[0] > 0 ? a.mass2[0] : 0.0;
+      const double sigma = a.sigma_pref * pow(2 * sqrt(k * k + m2c), -0.5);
+      const double omega0 = sqrt(k * k + m20);
+      uint32_t src = m;
+      int mode = 0;
+      if (kz == 0 || 2 * kz == N) {
+        const uint32_t px = (N - kx) % N, py = (N - ky) % N;
+        const uint32_t pm = (px * N + py) * Nh + kz;
+        if (pm == m)
+          mode = 2;
+        else if (pm < m) {
+          mode = 1;
+          src = pm;
+        }
+      }
+      double u[6];
+#pragma unroll
+      for (uint32_t d = 0; d < 3; ++d) {
+        U4 c;
+        c.v[0] = src;
+        c.v[1] = d;
+        c.v[2] = a.stream;
+        c.v[3] = 0u;
+        const U4 r = philox4x32_10(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+        u[2 * d] = u53(r.v[0], r.v[1]);
+        u[2 * d + 1] = u53(r.v[2], r.v[3]);
+      }
+      // Rayleigh amplitude with <R^2> = 2 (the |g|^2 of the complex normals of k_grf_modes)
+      const double amp = sigma * sqrt(-2.0 * log(1.0 - u[a.comp]));
+      const double twopi = 6.283185307179586477;
+      double s00, c00, s01, c01, s10, c10;
+      sincos(twopi * u[2], &s00, &c00);
+      sincos(twopi * u[3], &s01, &c01);
+      sincos(twopi * u[4], &s10, &c10);
+      double2 e00 = make_double2(c00, s00), e01 = make_double2(c01, s01), e10 = make_double2(c10, s10);
+      if (mode == 2) {
+        e01 = make_double2(e00.x, -e00.y);
+        e10 = e00;
+      }
+      // e11 = e01 * e10 * conj(e00)
+      const double2 t = make_double2(e01.x * e10.x - e01.y * e10.y, e01.x * e10.y + e01.y * e10.x);
+      double2 e11 = make_double2(t.x * e00.x + t.y * e00.y, t.y * e00.x - t.x * e00.y);
+      double2 ec0 = a.comp == 0 ? e00 : e10, ec1 = a.comp == 0 ? e01 : e11;
+      if (mode == 1) {
+        const double2 t0 = ec0;
+        ec0 = make_double2(ec1.x, -ec1.y);
+        ec1 = make_double2(t0.x, -t0.y);
+      }
+      ph = make_double2(0.5 * amp * (ec0.x + ec1.x), 0.5 * amp * (ec0.y + ec1.y));
+      // -i omega0 * d,  d = amp (ec0 - ec1) / 2
+      const double dr = 0.5 * amp * (ec0.x - ec1.x), di = 0.5 * amp * (ec0.y - ec1.y);
+      pp = make_double2(omega0 * di - a.aDot * ph.x, -omega0 * dr - a.aDot * ph.y);
+    }
+    ((double2 *)a.phik)[m] = ph;
+    ((double2 *)a.pik)[m] = pp;
+  }
+}
+
+// j0 = -dx^2 rho_Q with rho_Q = (2/dx) (fStar/omegaStar)^2 gQ Im[conj(pi) phi]
+// (mattercurrents.h:27-48), from the current complex-scalar arrays.
+__global__ void k_u1_source(const void *f0, const float *f0lo, const void *f1, const float *f1lo, const void *p0,
+                            const void *p1, int f32, double coef, double *out, uint32_t V)
+{
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    double a0, a1, q0, q1;
+    if (f32) {
+      a0 = (double)((const float *)f0)[i] + (double)f0lo[i];
+      a1 = (double)((const float *)f1)[i] + (double)f1lo[i];
+      q0 = (double)((const float *)p0)[i];
+      q1 = (double)((const float *)p1)[i];
+    } else {
+      a0 = ((const double *)f0)[i];
+      a1 = ((const double *)f1)[i];
+      q0 = ((const double *)p0)[i];
+      q1 = ((const double *)p1)[i];
+    }
+    out[i] = coef * (q0 * a1 - q1 * a0);
+  }
+}
+
+// Gauss constraint in Fourier space (u1initializer.h:86-104): pi_A,i = conj(k-_i) / |k-|^2 * j0,
+// k-_i = 1 - exp(-2 pi i n_i / N), zero mode zero.
+__global__ void k_gauss_solve(const double2 *j0, double2 *out, int dir, uint32_t N, uint32_t nmodes)
+{
+  const uint32_t Nh = N / 2 + 1;
+  const double w = 6.283185307179586477 / (double)N;
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nmodes; m += gridDim.x * blockDim.x) {
+    const uint32_t kz = m % Nh;
+    const uint32_t kxy = m / Nh;
+    const uint32_t kx = kxy / N, ky = kxy - kx * N;
+    const uint32_t n[3] = {kx, ky, kz};
+    double k2 = 0.0, cr = 0.0, ci = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double sn, cs;
+      sincos(w * (double)n[i], &sn, &cs);
+      const double re = 1.0 - cs, im = sn;  // k-_i = 1 - cos + i sin
+      k2 += re * re + im * im;
+      if (i == dir) {
+        cr = re;
+        ci = -im;  // conjugate
+      }
+    }
+    double2 r = make_double2(0.0, 0.0);
+    if (m != 0) {
+      const double2 j = j0[m];
+      r = make_double2((cr * j.x - ci * j.y) / k2, (cr * j.y + ci * j.x) / k2);
+    }
+    out[m] = r;
+  }
+}
+
 __global__ void k_grf_finish(const double *src, double scale, double add, void *dst, float *dstlo, int f32,
                              uint32_t V)
 {
@@ -237,6 +371,33 @@ hipError_t launch_grf_modes(const GrfArgs &a, int64_t n2max, hipStream_t s)
   uint32_t blocks = (a.nmodes + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_grf_modes, dim3(blocks), dim3(256), 0, s, a, n2max);
+  return hipGetLastError();
+}
+
+hipError_t launch_grf_cs_modes(const GrfCsArgs &a, int64_t n2max, hipStream_t s)
+{
+  uint32_t blocks = (a.nmodes + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_grf_cs_modes, dim3(blocks), dim3(256), 0, s, a, n2max);
+  return hipGetLastError();
+}
+
+hipError_t launch_u1_source(const void *f0, const float *f0lo, const void *f1, const float *f1lo, const void *p0,
+                            const void *p1, int f32, double coef, double *out, uint32_t V, hipStream_t s)
+{
+  uint32_t blocks = (V + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_u1_source, dim3(blocks), dim3(256), 0, s, f0, f0lo, f1, f1lo, p0, p1, f32, coef, out, V);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauss_solve(const void *j0, void *out, int dir, int N, hipStream_t s)
+{
+  const uint32_t nmodes = (uint32_t)N * N * (N / 2 + 1);
+  uint32_t blocks = (nmodes + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_gauss_solve, dim3(blocks), dim3(256), 0, s, (const double2 *)j0, (double2 *)out, dir,
+                     (uint32_t)N, nmodes);
   return hipGetLastError();
 }
 

@@ -172,7 +172,8 @@ public:
     for (size_t j = 0; j < Ns; ++j) x[j] = fldS0(j) / fStar;
     if constexpr (NCs > 0) {
       const double re = fldCS0(0).re, im = fldCS0(0).im;
-      x[kNormCSVar] = (re == 0 && im == 0) ? 1e-14 : std::sqrt(re * re + im * im) / fStar;
+      x[kNormCSVar] = (re == 0 && im == 0) ? std::sqrt(2.0) * (1e-14 * 0.70710678118654752)
+                                           : std::sqrt(re * re + im * im) / fStar;
     }
     return x;
   }
@@ -184,11 +185,22 @@ public:
     pot0 = potentialPolynomial().eval(x);
     setMasses(x, std::make_index_sequence<Ns>{});
     if constexpr (NCs > 0) {
-      Poly d2 = potentialPolynomial().derivative(kNormCSVar).derivative(kNormCSVar);
-      if constexpr (requires(R &r) { r.potDeriv2NormCS(Tag<0>{}); })
-        d2 = static_cast<R &>(*this).potDeriv2NormCS(Tag<0>{});
-      const double m2 = d2.eval(x);
-      masses2CS(0) = Complexify<T>(m2, m2);
+      // Potential::deriv2CS (definitions/potential.h:36-43,83-88) per component c:
+      //   V''(|phi|) phi_c^2/|phi|^2 + V'(|phi|)/|phi| (1 - phi_c^2/|phi|^2)
+      R &self = static_cast<R &>(*this);
+      Poly d1 = potentialPolynomial().derivative(kNormCSVar);
+      Poly d2 = d1.derivative(kNormCSVar);
+      if constexpr (requires(R &r) { r.potDerivNormCS(Tag<0>{}); }) d1 = Poly(self.potDerivNormCS(Tag<0>{}));
+      if constexpr (requires(R &r) { r.potDeriv2NormCS(Tag<0>{}); }) d2 = Poly(self.potDeriv2NormCS(Tag<0>{}));
+      const double r = x[kNormCSVar];
+      double c[2] = {fldCS0(0).re / fStar, fldCS0(0).im / fStar};
+      if (c[0] == 0 && c[1] == 0) c[0] = c[1] = 1e-14 * 0.70710678118654752;
+      double m2[2];
+      for (int k = 0; k < 2; ++k) {
+        const double w = c[k] * c[k] / (r * r);
+        m2[k] = d2.eval(x) * w + d1.eval(x) / r * (1 - w);
+      }
+      masses2CS(0) = Complexify<T>(m2[0], m2[1]);
     }
   }
   void setInitialPotentialFromPotential() { pot0 = potentialPolynomial().eval(initialPoint()); }

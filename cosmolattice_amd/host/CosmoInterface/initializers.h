@@ -1,5 +1,5 @@
 // Initial conditions with the reference's API (initializers/modelinitializer.h:47-81,
-// scalefactorinitializer.h:30-89, scalarsingletinitializer.h:41-77).  The Gaussian random fields are
+// scalefactorinitializer.h:30-89, scalarsingletinitializer.h:41-77, u1initializer.h:77-230).  The Gaussian random fields are
 // drawn on the GPU (clmi_grf_scalar) from Philox4x32-10 keyed by FNV-1a-64 of the seed string, so a
 // run is reproducible from its parameter file; the stream is not TempLat's.
 #pragma once
@@ -49,13 +49,14 @@ public:
       model.check(clmi_grf_scalar(ctx, (int)f, key, (uint32_t)f, model.masses2S((int)f), model.aDotI, model.kIR,
                                   rPar.kCutoff, model.fldS0((int)f) / fs, model.piS0((int)f) / fo),
                   "clmi_grf_scalar");
-    // complex scalar and gauge field: homogeneous condensate, zero gauge field
+    // complex scalar with constrained phases, gauge momenta from the Gauss constraint
+    // (u1initializer.h:77-117)
     if constexpr (Model::NCs > 0) {
-      const size_t V = (size_t)model.N * model.N * model.N;
-      for (int c = 0; c < 2; ++c) {
-        model.setField(CLMI_FLD_CS, 0, c, std::vector<double>(V, model.fldCS0(0)[c] / fs));
-        model.setField(CLMI_PI_CS, 0, c, std::vector<double>(V, model.piCS0(0)[c] / fo));
-      }
+      const double m2[2] = {model.masses2CS(0).re, model.masses2CS(0).im};
+      const double f0[2] = {model.fldCS0(0).re / fs, model.fldCS0(0).im / fs};
+      const double p0[2] = {model.piCS0(0).re / fo, model.piCS0(0).im / fo};
+      model.check(clmi_grf_u1(ctx, key, (uint32_t)(Model::Ns + 1), m2, model.aDotI, model.kIR, rPar.kCutoff, f0, p0),
+                  "clmi_grf_u1");
     }
     model.pushBackground();
     // setAllAverages, then aDotI from the Friedmann constraint (modelinitializer.h:71-80)

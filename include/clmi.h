@@ -193,6 +193,18 @@ int clmi_spectrum_bins(clmi_ctx *ctx, int kind, int index, int comp, double dk, 
 int clmi_grf_scalar(clmi_ctx *ctx, int index, uint64_t seed, uint32_t stream, double mass2, double aDot,
                     double kIR, double kCutOff, double fld0, double pi0);
 
+/* Complex scalar + U(1) initial conditions (U1Initializer::initializeRandomWithMatterU1,
+ * initializers/u1initializer.h:77-117, with addFluctuationsCScalarFromPhases, :157-230): vacuum
+ * fluctuations of both complex-scalar components built from Rayleigh amplitudes and phases
+ * e00, e01, e10 with e11 = e01 e10 conj(e00), so that every mode carries zero charge; then the
+ * homogeneous values fld0[c], pi0[c] (program units) are added.  With a U(1) field, A = 0 and
+ * pi_A = conj(k-_i)/|k-|^2 * FFT(-dx^2 rho_Q), k-_i = 1 - exp(-2 pi i n_i/N), which satisfies the
+ * Gauss constraint.  mass2[c]: effective mass^2 of component c (Potential::deriv2CS,
+ * definitions/potential.h:83-88); the momenta of both components use the frequency of component 0,
+ * as the reference does.  Random numbers: Philox4x32-10 keyed by seed, counter (mode, draw, stream). */
+int clmi_grf_u1(clmi_ctx *ctx, uint64_t seed, uint32_t stream, const double mass2[2], double aDot, double kIR,
+                double kCutOff, const double fld0[2], const double pi0[2]);
+
 /* ---- tuning and timing ------------------------------------------------------------------------ */
 
 /* No counterpart in the reference (tile shapes belong to this library's kernels).  Times every

@@ -367,3 +367,88 @@ def grf_scalar(N: int, dx: float, spec, seed: int, stream: int, mass2: float, aD
     phi = np.fft.irfftn(phik, s=(N, N, N), axes=(0, 1, 2)) + fld0
     pi = np.fft.irfftn(pik, s=(N, N, N), axes=(0, 1, 2)) + pi0
     return phi, pi
+
+
+# ---------------------------------------------------------------------------------------------
+# Complex scalar + U(1) initial conditions (initializers/u1initializer.h:77-117, 157-230)
+# ---------------------------------------------------------------------------------------------
+
+def _uniforms6(seed: int, stream: int, m):
+    z = np.zeros_like(m)
+    out = []
+    for d in range(3):
+        r0, r1, r2, r3 = philox4x32_10(m, z + d, z + stream, z, seed & 0xFFFFFFFF, seed >> 32)
+        out += [_u53(r0, r1), _u53(r2, r3)]
+    return out
+
+
+def grf_cs_modes(N: int, seed: int, stream: int, mass2, aDot: float, kIR: float, kCut: float, sigma_pref: float):
+    """Half-complex modes (phi_0, phi_1, pi_0, pi_1) of the complex scalar: Rayleigh amplitudes,
+    phases e00, e01, e10 and the constrained e11 = e01 e10 conj(e00) (u1initializer.h:157-230);
+    the mode -n on the kz = 0, N/2 planes uses the draws of n with (e_c0, e_c1) -> (conj e_c1,
+    conj e_c0); self-conjugate modes use e01 = conj e00, e10 = e00."""
+    Nh = N // 2 + 1
+    kx, ky, kz = np.meshgrid(np.arange(N), np.arange(N), np.arange(Nh), indexing="ij")
+    m = ((kx * N + ky) * Nh + kz).astype(np.uint64)
+    nx = np.where(kx <= N // 2, kx, kx - N)
+    ny = np.where(ky <= N // 2, ky, ky - N)
+    n2 = nx * nx + ny * ny + kz * kz
+    k = np.sqrt(n2.astype(np.float64)) * kIR
+    edge = (kz == 0) | (2 * kz == N)
+    pm = ((((N - kx) % N) * N + (N - ky) % N) * Nh + kz).astype(np.uint64)
+    self_conj = edge & (pm == m)
+    partner = edge & (pm < m)
+    src = np.where(partner, pm, m)
+    u = _uniforms6(seed, stream, src)
+    e00 = np.exp(2j * np.pi * u[2])
+    e01 = np.exp(2j * np.pi * u[3])
+    e10 = np.exp(2j * np.pi * u[4])
+    e01 = np.where(self_conj, np.conj(e00), e01)
+    e10 = np.where(self_conj, e00, e10)
+    e11 = e01 * e10 * np.conj(e00)
+    omega0 = np.sqrt(k * k + max(mass2[0], 0.0))
+    keep = (m != 0) & (n2 <= n2max_for_cut(kIR, kCut, N))
+    out = []
+    for comp, (ea, eb) in enumerate(((e00, e01), (e10, e11))):
+        with np.errstate(divide="ignore"):
+            sigma = sigma_pref * np.power(2 * np.sqrt(k * k + max(mass2[comp], 0.0)), -0.5)
+        amp = sigma * np.sqrt(-2.0 * np.log(1.0 - u[comp]))
+        c0 = np.where(partner, np.conj(eb), ea)
+        c1 = np.where(partner, np.conj(ea), eb)
+        phik = 0.5 * amp * (c0 + c1)
+        pik = -1j * omega0 * 0.5 * amp * (c0 - c1) - aDot * phik
+        out.append((np.where(keep, phik, 0), np.where(keep, pik, 0)))
+    return out
+
+
+def gauss_solve_modes(j0k, N: int):
+    """pi_A,i modes from the Gauss constraint: conj(k-_i)/|k-|^2 j0, k-_i = 1 - exp(-2 pi i n_i/N)
+    (u1initializer.h:86-104); zero mode zero."""
+    Nh = N // 2 + 1
+    n = np.meshgrid(np.arange(N), np.arange(N), np.arange(Nh), indexing="ij")
+    km = [1.0 - np.exp(-2j * np.pi * ni / N) for ni in n]
+    k2 = sum(np.abs(x) ** 2 for x in km)
+    k2[0, 0, 0] = 1.0
+    out = []
+    for i in range(3):
+        p = np.conj(km[i]) / k2 * j0k
+        p[0, 0, 0] = 0
+        out.append(p)
+    return out
+
+
+def grf_u1(N: int, dx: float, spec, seed: int, stream: int, mass2, aDot: float, kIR: float, kCut: float, fld0,
+           pi0):
+    """Real-space complex-scalar components (phi_0, phi_1, pi_0, pi_1) and pi_A (3 components) of
+    the U(1) initial conditions; A = 0."""
+    L = N * dx
+    sigma_pref = spec.omegaStar / spec.fStar * (L / dx ** 2) ** 1.5 / math.sqrt(2.0)
+    modes = grf_cs_modes(N, seed, stream, mass2, aDot, kIR, kCut, sigma_pref)
+    s = (N, N, N)
+    phi = [np.fft.irfftn(modes[c][0], s=s, axes=(0, 1, 2)) + fld0[c] for c in range(2)]
+    pi = [np.fft.irfftn(modes[c][1], s=s, axes=(0, 1, 2)) + pi0[c] for c in range(2)]
+    fw = spec.fStar / spec.omegaStar
+    rho = 2.0 / dx * fw * fw * spec.gQ * (pi[0] * phi[1] - pi[1] * phi[0])
+    j0k = np.fft.rfftn(-dx * dx * rho, axes=(0, 1, 2))
+    piA = [np.fft.irfftn(p, s=s, axes=(0, 1, 2)) for p in gauss_solve_modes(j0k, N)]
+    return phi, pi, piA

@@ -151,3 +151,35 @@ def test_host_initializer_matches_python_path(gpu, oracle_mod, tmp_path):
     assert bg["aDotI"] == pytest.approx(math.sqrt(rhs), rel=1e-14)
     assert bg["aI"] == 1.0
     lat.close()
+
+
+def test_host_lphi4u1_initializer(gpu, oracle_mod, tmp_path):
+    """The lphi4U1 model program's initializer draws the complex scalar with constrained phases and
+    the gauge momenta from the Gauss constraint: the fields equal the numpy restatement (stream 1,
+    key FNV-1a-64 of the seed string) and the Gauss law holds."""
+    say("case host lphi4U1 initializer")
+    from cosmolattice_amd.clmi import FLD_CS, PI_CS, PI_U1, lphi4U1
+
+    N, kIR, dt = 32, 0.4, 0.005
+    dx = 2 * math.pi / kIR / N
+    phi0 = 5.6964e18
+    g = math.sqrt(10.0)
+    params = (f"N = {N}\nkIR = {kIR!r}\ndt = {dt!r}\ntMax = 0\ntOutputFreq = {dt!r}\ntOutputInfreq = {dt!r}\n"
+              f"tOutputRareFreq = {dt!r}\nexpansion = false\nlambda = 1\ngU1s = {g!r}\nCSU1_charges = 1\n"
+              f"cmplx_field_initial_norm = {phi0!r}\ncmplx_momentum_initial_norm = 0\nbaseSeed = 3\n"
+              f"kCutOff = {N * kIR / 4!r}\n")
+    run_program("lphi4U1", tmp_path, params, f"clmi_dump_dir={tmp_path}")
+    keys = [(FLD_CS, 0, 0), (FLD_CS, 0, 1), (PI_CS, 0, 0), (PI_CS, 0, 1)] + [(PI_U1, 0, d) for d in range(3)]
+    got, _ = read_state(tmp_path, keys, N)
+    spec = lphi4U1(10.0, phi0_gev=phi0, lam=1.0, pi0_gev2=0.0)
+    m2 = spec.cs_mass2(spec.fld0)
+    phi, pi, piA = oracle_mod.grf_u1(N, dx, spec, fnv1a64("3"), 1, m2, 0.0, kIR, N * kIR / 4, spec.fld0, spec.pi0)
+    for c in range(2):
+        assert np.max(np.abs(got[(FLD_CS, 0, c)] - phi[c])) < 1e-12 * np.max(np.abs(phi[c] - spec.fld0[c]))
+        assert np.max(np.abs(got[(PI_CS, 0, c)] - pi[c])) < 1e-12 * np.max(np.abs(pi[c]))
+    for d in range(3):
+        assert np.max(np.abs(got[(PI_U1, 0, d)] - piA[d])) < 1e-12 * np.max(np.abs(piA[d]))
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    for (k, i, c), a in got.items():
+        orc.upload(k, i, c, a)
+    assert orc.gauss_u1()[0] < 1e-13

@@ -108,7 +108,8 @@ def test_lphi4u1_header(tmp_path):
     assert c["cs_term"] == [[1.0, 4.0]]  # V = |phi|^4 (models/lphi4U1.h:117-120)
     assert c["gQ"] == pytest.approx(9.486832980505138e-07, rel=1e-15)  # charge 1 x gU1s
     assert c["pot0"] == pytest.approx(1.0, rel=1e-14)  # |phi| = 1 in program units
-    assert c["mass2cs"] == pytest.approx(12.0, rel=1e-14)  # potDeriv2NormCS = 12 |phi|^2
+    # deriv2CS per component at phi_0 = phi_1 = 1/sqrt2: V'' phi_c^2/|phi|^2 + V'/|phi| (1 - phi_c^2/|phi|^2)
+    assert c["mass2cs"] == pytest.approx(12.0 * 0.5 + 4.0 * 0.5, rel=1e-14)
 
 
 @pytest.mark.skipif(not (REFERENCE_MODELS / "lphi4.h").exists(), reason="reference tree not present")

@@ -465,3 +465,79 @@ def test_grf_statistics_and_structure(oracle_mod):
     np.testing.assert_allclose(back, phik, atol=1e-9 * np.abs(phik).max())
     assert phi.mean() == pytest.approx(0.25, abs=1e-12)
     assert pi.mean() == pytest.approx(-0.5, abs=1e-12)
+
+
+# ------------------------------------------------------------------------------------------------
+# Complex scalar + U(1) initial conditions
+# ------------------------------------------------------------------------------------------------
+
+def u1_spec(gQ=3.0):
+    spec = lphi4U1(10.0, phi0_gev=5.6964e18, lam=1.0, pi0_gev2=0.0)
+    spec.gQ = gQ
+    return spec
+
+
+def test_u1_ics_satisfy_gauss_law_and_carry_no_charge(oracle_mod):
+    """The constrained phases make every mode's charge vanish, and pi_A from the Fourier-space
+    Gauss solve satisfies the lattice Gauss law (backward differences) at every site; the law then
+    stays satisfied under the leapfrog."""
+    N, kIR = 16, 0.5
+    dx = 2 * math.pi / kIR / N
+    spec = u1_spec()
+    m2 = spec.cs_mass2(spec.fld0)
+    phi, pi, piA = oracle_mod.grf_u1(N, dx, spec, 11, 1, m2, 0.2, kIR, 1e9, spec.fld0, spec.pi0)
+    rho = pi[0] * phi[1] - pi[1] * phi[0]
+    assert abs(rho.sum()) < 1e-13 * np.abs(rho).sum()
+    lat = oracle_mod.OracleLattice(spec, N, dx, 0.01, expansion=False)
+    for c in range(2):
+        lat.upload(FLD_CS, 0, c, phi[c])
+        lat.upload(PI_CS, 0, c, pi[c])
+    for d in range(3):
+        lat.upload(PI_U1, 0, d, piA[d])
+    g = lat.gauss_u1()
+    assert g[1] > 1.0 and g[0] < 1e-14
+    # a deliberately wrong solve (forward instead of backward lattice momentum) violates the law
+    j0k = np.fft.rfftn(-dx * dx * 2.0 / dx * (spec.fStar / spec.omegaStar) ** 2 * spec.gQ * rho)
+    wrong = [np.fft.irfftn(np.conj(p), s=(N, N, N), axes=(0, 1, 2)) for p in oracle_mod.gauss_solve_modes(np.conj(j0k), N)]
+    for d in range(3):
+        lat.upload(PI_U1, 0, d, wrong[d])
+    assert lat.gauss_u1()[0] > 1e-3
+    for d in range(3):
+        lat.upload(PI_U1, 0, d, piA[d])
+    lat.lf_init()
+    lat.lf_evolve(20)
+    lat.lf_sync()
+    assert lat.gauss_u1()[0] < 1e-12
+
+
+def test_u1_ics_statistics_and_structure(oracle_mod):
+    """E|phi_c,k|^2 = sigma_c^2 (half a real scalar's power: phi = (phi_0 + i phi_1)/sqrt2),
+    E|pi_c,k + aDot phi_c,k|^2 = omega_0^2 sigma_c^2 with component 0's frequency for both, zero
+    mode and modes beyond the cut are zero, and the real fields reproduce the half-space modes."""
+    N, kIR, aDot = 16, 0.3, 0.4
+    dx = 2 * math.pi / kIR / N
+    L = N * dx
+    pref = (L / dx ** 2) ** 1.5 / math.sqrt(2)
+    m2 = [1.7, 0.6]
+    kcut = 6.5 * kIR
+    modes = oracle_mod.grf_cs_modes(N, 5, 2, m2, aDot, kIR, kcut, pref)
+    Nh = N // 2 + 1
+    kx, ky, kz = np.meshgrid(np.arange(N), np.arange(N), np.arange(Nh), indexing="ij")
+    nx = np.where(kx <= N // 2, kx, kx - N)
+    ny = np.where(ky <= N // 2, ky, ky - N)
+    n2 = nx ** 2 + ny ** 2 + kz ** 2
+    k = np.sqrt(n2) * kIR
+    keep = (n2 > 0) & (k <= kcut)
+    inner = keep & (kz > 0) & (2 * kz < N)
+    om0 = np.sqrt(k ** 2 + m2[0])
+    for c in range(2):
+        phik, pik = modes[c]
+        assert np.all(phik[~keep] == 0) and np.all(pik[~keep] == 0)
+        sig2 = pref ** 2 / (2 * np.sqrt(k ** 2 + m2[c]))
+        r = (np.abs(phik) ** 2 / sig2)[inner]
+        assert abs(r.mean() - 1) < 5 / math.sqrt(r.size)
+        d = pik + aDot * phik
+        r2 = (np.abs(d) ** 2 / (om0 ** 2 * sig2))[inner]
+        assert abs(r2.mean() - 1) < 5 / math.sqrt(r2.size)
+        back = np.fft.rfftn(np.fft.irfftn(phik, s=(N, N, N), axes=(0, 1, 2)), axes=(0, 1, 2))
+        np.testing.assert_allclose(back, phik, atol=1e-12 * np.abs(phik).max())
