@@ -29,6 +29,7 @@ HIP_SOURCES = [
     "gauge_pass.hip",
     "background.hip",
     "fft_kernels.hip",
+    "adjoint_kernels.hip",
     "clmi_api.hip",
 ]
 

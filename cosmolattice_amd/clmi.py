@@ -115,6 +115,9 @@ def load_library(path: Path | None = None) -> C.CDLL:
                                       C.c_double, C.c_double, C.c_double]),
         "clmi_grf_u1": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_double,
                                   C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "clmi_adjoint_lf": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_double), C.POINTER(Background)]),
         "clmi_profile_enable": (C.c_int, [P, C.c_int]),
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
@@ -403,6 +406,30 @@ class Lattice:
         f = (C.c_double * 2)(*fld0)
         p = (C.c_double * 2)(*pi0)
         self._check(self.lib.clmi_grf_u1(self._h, seed, stream, m, aDot, kIR, kCutOff, f, p), "grf_u1")
+
+    def adjoint_lf(self, nsteps: int, seed_fld: Sequence[np.ndarray], seed_pi: Sequence[np.ndarray],
+                   seed_bg: Sequence[float] = (0.0, 0.0), checkpoint_every: int = 0, bg_from_fields: bool = True):
+        """Reverse-mode derivative of [initial background] -> lf_init -> nsteps steps -> sync from the
+        current state (clmi_adjoint_lf).  Returns (dL/dphi_0 per field, dL/dpi_0 per field,
+        dL/dterm_coeff, dL/d initial background dict); the lattice ends at the forward's final state."""
+        nf = self.spec.n_scalars
+        V = self.N ** 3
+        seeds_f = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1)) for a in seed_fld]
+        seeds_p = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1)) for a in seed_pi]
+        if len(seeds_f) != nf or len(seeds_p) != nf or any(a.size != V for a in seeds_f + seeds_p):
+            raise ValueError("one N^3 seed array per scalar field is required")
+        gf = [np.empty(V) for _ in range(nf)]
+        gp = [np.empty(V) for _ in range(nf)]
+        ptrs = lambda arrs: (C.c_void_p * nf)(*[a.ctypes.data for a in arrs])  # noqa: E731
+        sb = (C.c_double * 2)(*seed_bg)
+        gc = (C.c_double * max(1, len(self.spec.terms)))()
+        gb = Background()
+        self._check(self.lib.clmi_adjoint_lf(self._h, nsteps, checkpoint_every, 1 if bg_from_fields else 0,
+                                             ptrs(seeds_f), ptrs(seeds_p), sb, ptrs(gf), ptrs(gp), gc, C.byref(gb)),
+                    "adjoint_lf")
+        shape = (self.N,) * 3
+        return ([a.reshape(shape) for a in gf], [a.reshape(shape) for a in gp], list(gc)[:len(self.spec.terms)],
+                gb.as_dict())
 
     # -- timing ------------------------------------------------------------------------------------
     def profile_enable(self, on: bool = True):

@@ -12,6 +12,7 @@
 // deferred drift with a drift-only pass.
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -969,6 +970,328 @@ int clmi_grf_u1(clmi_ctx *c, uint64_t seed, uint32_t stream, const double mass2[
       HIPCHK(c, launch_grf_finish(c->stage, scale, 0.0, P.p, nullptr, c->f32, c->V, c->stream));
     }
   }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---- adjoint of the leapfrog (scalar singlets) ----------------------------------------------
+
+// Device copy of one leapfrog state (phi, pi of every scalar singlet) in storage precision.
+struct AdjState {
+  DevArr f[CLMI_MAX_SCALARS], p[CLMI_MAX_SCALARS];
+};
+
+int adj_alloc(clmi_ctx *c, AdjState &s)
+{
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    CHK(alloc_arr(c, s.f[f], true));
+    CHK(alloc_arr(c, s.p[f], false));
+  }
+  return CLMI_OK;
+}
+void adj_free(AdjState &s)
+{
+  for (int f = 0; f < CLMI_MAX_SCALARS; ++f) {
+    free_arr(s.f[f]);
+    free_arr(s.p[f]);
+  }
+}
+int adj_copy(clmi_ctx *c, const DevArr &src, DevArr &dst, bool field)
+{
+  const size_t b = bytes_field(c->f32) * c->V;
+  HIPCHK(c, hipMemcpyAsync(dst.p, src.p, b, hipMemcpyDeviceToDevice, c->stream));
+  if (field && c->f32)
+    HIPCHK(c, hipMemcpyAsync(dst.lo, src.lo, sizeof(float) * c->V, hipMemcpyDeviceToDevice, c->stream));
+  return CLMI_OK;
+}
+int adj_save(clmi_ctx *c, AdjState &s)
+{
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    CHK(adj_copy(c, c->fS[f][c->cur_fS], s.f[f], true));
+    CHK(adj_copy(c, c->pS[f][c->cur_pS], s.p[f], false));
+  }
+  return CLMI_OK;
+}
+int adj_restore(clmi_ctx *c, const AdjState &s)
+{
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    CHK(adj_copy(c, s.f[f], c->fS[f][c->cur_fS], true));
+    CHK(adj_copy(c, s.p[f], c->pS[f][c->cur_pS], false));
+  }
+  return CLMI_OK;
+}
+AdjFields adj_view(const clmi_ctx *c, const AdjState &s)
+{
+  AdjFields v{};
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    v.phi[f] = s.f[f].p;
+    v.philo[f] = s.f[f].lo;
+    v.pi[f] = s.p[f].p;
+  }
+  return v;
+}
+
+// ScaleFactorKernels::get for scalar singlets and its partial derivatives.
+struct SFd {
+  double K, dpi2, dG, dU, da;
+};
+SFd sf_partials(const ModelConsts &mc, double a, double pi2, double G, double U)
+{
+  const double al = mc.alpha, fm = mc.fStar / mc.MPl;
+  const double A = std::pow(a, 2 * al + 1) / 3.0 * fm * fm;
+  const double B = (al - 2) * 0.5 * pi2 * std::pow(a, -6.0) + al * 0.5 * G * std::pow(a, -2.0) + (al + 1) * U;
+  SFd d;
+  d.K = A * B;
+  d.dpi2 = A * (al - 2) * 0.5 * std::pow(a, -6.0);
+  d.dG = A * al * 0.5 * std::pow(a, -2.0);
+  d.dU = A * (al + 1);
+  d.da = (2 * al + 1) / a * A * B +
+         A * ((al - 2) * 0.5 * pi2 * (-6.0) * std::pow(a, -7.0) + al * 0.5 * G * (-2.0) * std::pow(a, -3.0));
+  return d;
+}
+
+struct AdjRun {
+  clmi_ctx *c = nullptr;
+  std::vector<AdjState> checkpoints, segment;
+  AdjState final_state;
+  double *lam = nullptr;       // 2 * n_scalars FP64 arrays
+  double *partials = nullptr;  // 1024 x kAdjRed
+  double *red = nullptr;       // kAdjRed
+  ~AdjRun()
+  {
+    for (auto &s : checkpoints) adj_free(s);
+    for (auto &s : segment) adj_free(s);
+    adj_free(final_state);
+    if (lam) (void)hipFree(lam);
+    if (partials) (void)hipFree(partials);
+    if (red) (void)hipFree(red);
+  }
+};
+
+int adj_read_red(clmi_ctx *c, const AdjRun &r, double out[kAdjRed])
+{
+  HIPCHK(c, hipMemcpyAsync(out, r.red, sizeof(double) * kAdjRed, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CLMI_OK;
+}
+
+// One forward leapfrog step exactly as clmi_lf_evolve runs it, then the deferred drift, so the
+// context holds a materialised state again.
+int adj_forward_step(clmi_ctx *c)
+{
+  CHK(clmi_lf_evolve(c, 1));
+  CHK(flush(c));
+  return CLMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_fields,
+                    const double *const *seed_fld, const double *const *seed_pi, const double seed_bg[2],
+                    double *const *grad_fld, double *const *grad_pi, double *grad_coeff, clmi_background *grad_bg)
+{
+  CHK(check_ctx(c));
+  const int nf = c->cfg.n_scalars;
+  if (nf < 1 || c->cfg.n_cscalars != 0) return fail(c, CLMI_E_INVALID, "the adjoint covers scalar-singlet models");
+  if (nsteps < 1) return fail(c, CLMI_E_INVALID, "nsteps must be >= 1");
+  if (!seed_fld || !seed_pi || !seed_bg || !grad_fld || !grad_pi || !grad_coeff)
+    return fail(c, CLMI_E_INVALID, "null argument");
+  for (int f = 0; f < nf; ++f)
+    if (!seed_fld[f] || !seed_pi[f] || !grad_fld[f] || !grad_pi[f]) return fail(c, CLMI_E_INVALID, "null array");
+  const int M = checkpoint_every > 0 ? checkpoint_every
+                                     : std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+  const bool ex = c->cfg.expansion != 0;
+  const double dt = c->cfg.dt, al = c->cfg.alpha;
+  const double V = (double)c->V;
+  const ModelConsts &mc = c->mc;
+  CHK(flush(c));
+
+  AdjRun run;
+  run.c = c;
+  const int ncp = (nsteps + M - 1) / M;
+  run.checkpoints.resize(ncp);
+  run.segment.resize(M + 1);
+  for (auto &s : run.checkpoints) CHK(adj_alloc(c, s));
+  for (auto &s : run.segment) CHK(adj_alloc(c, s));
+  CHK(adj_alloc(c, run.final_state));
+  HIPCHK(c, hipMalloc(&run.lam, sizeof(double) * 2 * nf * c->V));
+  HIPCHK(c, hipMalloc(&run.partials, sizeof(double) * 1024 * kAdjRed));
+  HIPCHK(c, hipMalloc(&run.red, sizeof(double) * kAdjRed));
+
+  // ---- forward: initial background, lf_init, nsteps steps, sync; record the background tape ----
+  std::vector<clmi_background> tape(nsteps + 1);
+  clmi_background b_init{}, b_fin{};
+  if (bg_from_fields) {
+    // ModelInitializer: setAllAverages, then aDotI = sqrt(Friedmann RHS) (modelinitializer.h:71-80)
+    double hc[3];
+    CHK(clmi_hubble_constraint(c, hc));
+    CHK(clmi_get_background(c, &b_init));
+    b_init.aDotI = std::sqrt(hc[2]);
+    CHK(clmi_set_background(c, &b_init));
+  }
+  CHK(clmi_get_background(c, &b_init));
+  CHK(clmi_lf_init(c, 0));
+  CHK(clmi_get_background(c, &tape[0]));
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % M == 0) CHK(adj_save(c, run.checkpoints[n / M]));
+    CHK(adj_forward_step(c));
+    CHK(clmi_get_background(c, &tape[n + 1]));
+  }
+  CHK(clmi_lf_sync(c));
+  CHK(clmi_get_background(c, &b_fin));
+  CHK(adj_save(c, run.final_state));
+
+  // ---- reverse ----
+  AdjArgs a{};
+  a.nf = nf;
+  a.N = (uint32_t)c->N;
+  a.V = c->V;
+  a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
+  a.poly = c->spoly;
+  a.partials = run.partials;
+  a.red_out = run.red;
+  for (int f = 0; f < nf; ++f) {
+    a.lam_phi[f] = run.lam + (size_t)f * c->V;
+    a.lam_pi[f] = run.lam + (size_t)(nf + f) * c->V;
+    HIPCHK(c, hipMemcpyAsync(a.lam_phi[f], seed_fld[f], sizeof(double) * c->V, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(a.lam_pi[f], seed_pi[f], sizeof(double) * c->V, hipMemcpyHostToDevice, c->stream));
+  }
+  const int nt = c->spoly.nterms;
+  std::vector<double> gc(nt, 0.0);
+  double red[kAdjRed];
+  // scalar adjoints of the current background state
+  double lam_aI = seed_bg[0], lam_aDotSI = 0, lam_pi2SI = 0, lam_G = 0, lam_U = 0;
+
+  // final sync (leapfrog.h:96-116): pi_T = pi + dt/2 kernel(phi_T); pi2AvI = <pi_T^2>;
+  // aDotI = aDotSI + dt/2 K
+  {
+    const clmi_background &bT = tape[nsteps];
+    double lam_pi2T = 0.0;
+    if (ex) {
+      lam_aDotSI = seed_bg[1];
+      const double lamK = 0.5 * dt * seed_bg[1];
+      const SFd d = sf_partials(mc, bT.aI, b_fin.pi2AvI, bT.grad2AvI, bT.potAvI);
+      lam_pi2T = lamK * d.dpi2;
+      lam_G = lamK * d.dG;
+      lam_U = lamK * d.dU;
+      lam_aI += lamK * d.da;
+    }
+    a.fwd = adj_view(c, run.final_state);
+    HIPCHK(c, launch_adj_pi(a, c->f32, 0.0, lam_pi2T, c->stream));
+    const double aK = std::pow(bT.aI, 1 + al), aV = std::pow(bT.aI, 3 + al);
+    HIPCHK(c, launch_adj_kick(a, c->f32, 0.5 * dt, aK, aV, c->stream));
+    CHK(adj_read_red(c, run, red));
+    if (ex) lam_aI += 0.5 * dt * (red[0] * (1 + al) * std::pow(bT.aI, al) - red[1] * (3 + al) * std::pow(bT.aI, 2 + al));
+    for (int t = 0; t < nt; ++t) gc[t] += -0.5 * dt * aV * red[2 + t];
+  }
+
+  for (int k = ncp - 1; k >= 0; --k) {
+    const int n0 = k * M, n1 = std::min(nsteps, n0 + M);
+    // recompute the segment's states n0..n1 from checkpoint k
+    CHK(adj_restore(c, run.checkpoints[k]));
+    CHK(clmi_set_background(c, &tape[n0]));
+    c->synced = (n0 == 0);
+    c->pending_drift = false;
+    for (int n = n0; n < n1; ++n) {
+      CHK(adj_save(c, run.segment[n - n0]));
+      CHK(adj_forward_step(c));
+    }
+    CHK(adj_save(c, run.segment[n1 - n0]));
+    for (int n = n1 - 1; n >= n0; --n) {
+      const clmi_background &bn = tape[n], &bn1 = tape[n + 1];
+      const double w = (n == 0) ? 0.5 : 1.0;
+      // storeFieldsAverages of phi_{n+1}, then the drift's sum lam_phi . pi_{n+1/2}
+      a.fwd = adj_view(c, run.segment[n + 1 - n0]);
+      HIPCHK(c, launch_adj_fields(a, c->f32, lam_G, lam_U, 0.0, false, c->stream));
+      CHK(adj_read_red(c, run, red));
+      for (int t = 0; t < nt; ++t) gc[t] += lam_U * red[1 + t] / V;
+      const double F = std::pow(bn1.aSI, al - 3);
+      double lamP = 0.0;
+      double lam_aI_n = 0, lam_aDotSI_n = 0, lam_pi2SI_n = 0, lam_G_n = 0, lam_U_n = 0;
+      if (ex) {
+        const double lam_aSI1 = dt * red[0] * (al - 3) * std::pow(bn1.aSI, al - 4);
+        // aSI' = (aI + aI')/2; aI' = aI + dt aDotSI'; aDotSI' = aDotSI + dt K
+        double lam_aI1 = lam_aI + 0.5 * lam_aSI1;
+        lam_aI_n = 0.5 * lam_aSI1 + lam_aI1;
+        const double lam_aDotSI1 = lam_aDotSI + dt * lam_aI1;
+        lam_aDotSI_n = lam_aDotSI1;
+        const double lamK = dt * lam_aDotSI1;
+        const SFd d = sf_partials(mc, bn.aI, bn1.pi2AvI, bn.grad2AvI, bn.potAvI);
+        const double lam_pi2I = lamK * d.dpi2;
+        lam_G_n = lamK * d.dG;
+        lam_U_n = lamK * d.dU;
+        lam_aI_n += lamK * d.da;
+        // pi2AvI = (pi2AvSI_old + P)/2, pi2AvSI' = P
+        lam_pi2SI_n = 0.5 * lam_pi2I;
+        lamP = 0.5 * lam_pi2I + lam_pi2SI;
+      }
+      HIPCHK(c, launch_adj_pi(a, c->f32, dt * F, lamP, c->stream));
+      // kick of step n on phi_n
+      a.fwd = adj_view(c, run.segment[n - n0]);
+      const double aK = std::pow(bn.aI, 1 + al), aV = std::pow(bn.aI, 3 + al), ck = w * dt;
+      HIPCHK(c, launch_adj_kick(a, c->f32, ck, aK, aV, c->stream));
+      CHK(adj_read_red(c, run, red));
+      if (ex) lam_aI_n += ck * (red[0] * (1 + al) * std::pow(bn.aI, al) - red[1] * (3 + al) * std::pow(bn.aI, 2 + al));
+      for (int t = 0; t < nt; ++t) gc[t] += -ck * aV * red[2 + t];
+      if (ex) {
+        lam_aI = lam_aI_n;
+        lam_aDotSI = lam_aDotSI_n;
+        lam_pi2SI = lam_pi2SI_n;
+        lam_G = lam_G_n;
+        lam_U = lam_U_n;
+      }
+    }
+  }
+
+  // lf_init (leapfrog.h:35): aDotSI = aDotI - dt/2 K(initial averages)
+  clmi_background gb{};
+  if (ex) {
+    const double lam_aDotI0 = lam_aDotSI;
+    const double lamK0 = -0.5 * dt * lam_aDotSI;
+    const SFd d = sf_partials(mc, b_init.aI, b_init.pi2AvI, b_init.grad2AvI, b_init.potAvI);
+    double lam_pi2I0 = lamK0 * d.dpi2;
+    lam_G += lamK0 * d.dG;
+    lam_U += lamK0 * d.dU;
+    lam_aI += lamK0 * d.da;
+    gb.aI = lam_aI;
+    if (bg_from_fields) {
+      // aDotI = sqrt(a^(2 alpha + 2)/3 (fStar/MPl)^2 rho), rho = pi2/2 a^-6 + G/2 a^-2 + U, with
+      // pi2AvI = pi2AvSI = <pi^2>, grad2AvI = <|grad phi|^2>, potAvI = <V> of the initial fields
+      const double a0 = b_init.aI, fm = mc.fStar / mc.MPl;
+      const double lamRHS = lam_aDotI0 / (2.0 * b_init.aDotI);
+      const double lamRho = lamRHS * std::pow(a0, 2 * al + 2) / 3.0 * fm * fm;
+      const double lamP0 = lam_pi2I0 + lam_pi2SI + lamRho * 0.5 * std::pow(a0, -6.0);
+      const double lamG0 = lam_G + lamRho * 0.5 * std::pow(a0, -2.0);
+      const double lamU0 = lam_U + lamRho;
+      a.fwd = adj_view(c, run.checkpoints[0]);
+      HIPCHK(c, launch_adj_fields(a, c->f32, lamG0, lamU0, lamP0, true, c->stream));
+      CHK(adj_read_red(c, run, red));
+      for (int t = 0; t < nt; ++t) gc[t] += lamU0 * red[1 + t] / V;
+    } else {
+      gb.aDotI = lam_aDotI0;
+      gb.pi2AvI = lam_pi2I0;
+      gb.pi2AvSI = lam_pi2SI;
+      gb.grad2AvI = lam_G;
+      gb.potAvI = lam_U;
+    }
+  }
+  for (int f = 0; f < nf; ++f) {
+    HIPCHK(c, hipMemcpyAsync(grad_fld[f], a.lam_phi[f], sizeof(double) * c->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(grad_pi[f], a.lam_pi[f], sizeof(double) * c->V, hipMemcpyDeviceToHost, c->stream));
+  }
+  for (int t = 0; t < nt; ++t) grad_coeff[t] = gc[t];
+  if (grad_bg) *grad_bg = gb;
+  // leave the context at the end of the forward run
+  CHK(adj_restore(c, run.final_state));
+  CHK(clmi_set_background(c, &b_fin));
+  c->synced = true;
+  c->pending_drift = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLMI_OK;
 }

@@ -164,6 +164,29 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
   }
 }
 
+// ---- adjoint (adjoint_kernels.hip) ----
+constexpr int kAdjRed = 16;  // partial-sum slots of the adjoint reductions
+struct AdjFields {           // forward fields of one leapfrog state, in storage precision
+  const void *phi[CLMI_MAX_SCALARS];
+  const float *philo[CLMI_MAX_SCALARS];
+  const void *pi[CLMI_MAX_SCALARS];
+};
+struct AdjArgs {
+  AdjFields fwd;
+  double *lam_phi[CLMI_MAX_SCALARS];  // FP64 adjoint fields
+  double *lam_pi[CLMI_MAX_SCALARS];
+  int nf;
+  uint32_t N, V;
+  double inv_dx2;
+  ScalarPoly poly;
+  double *partials;  // >= 1024 rows of kAdjRed
+  double *red_out;   // kAdjRed finished sums
+};
+hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lamU, double lamP, bool with_pi,
+                             hipStream_t s);
+hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s);
+hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s);
+
 // Launchers (defined in the .hip files).
 struct TiledPlan {
   int XC = 0;      // planes per workgroup (x-chunk)

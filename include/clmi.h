@@ -205,6 +205,25 @@ int clmi_grf_scalar(clmi_ctx *ctx, int index, uint64_t seed, uint32_t stream, do
 int clmi_grf_u1(clmi_ctx *ctx, uint64_t seed, uint32_t stream, const double mass2[2], double aDot, double kIR,
                 double kCutOff, const double fld0[2], const double pi0[2]);
 
+/* ---- adjoint ----------------------------------------------------------------------------------- */
+
+/* No counterpart in the reference (a reverse-mode derivative of its leapfrog).  Runs, from the state
+ * the context holds, the forward map
+ *     [bg_from_fields: setAllAverages, aDotI = sqrt(Friedmann RHS)  (modelinitializer.h:71-80)]
+ *     LeapFrog constructor, nsteps x evolve, sync          (evolvers/leapfrog.h:29-116)
+ * and then its exact reverse-mode derivative.  Given the adjoint seeds of the outputs -- seed_fld[f]
+ * = dL/dphi_f(T), seed_pi[f] = dL/dpi_f(T) (synced momenta), seed_bg = {dL/daI, dL/daDotI} at T --
+ * it returns dL/dphi_f(0), dL/dpi_f(0), dL/dterm_coeff[t] for every potential monomial, and (when
+ * bg_from_fields is 0) the derivatives with respect to the initial background scalars aI, aDotI,
+ * pi2AvI, pi2AvSI, grad2AvI, potAvI in grad_bg (the other members zero).  Arrays are N^3 doubles on
+ * the host, one per scalar singlet.  The forward states are recomputed from checkpoints every
+ * checkpoint_every steps (<= 0: about sqrt(nsteps)); the adjoint fields are FP64 in both precision
+ * modes.  On return the context holds the forward run's final (synced) state.  Scalar-singlet models
+ * only; expansion on or off. */
+int clmi_adjoint_lf(clmi_ctx *ctx, int nsteps, int checkpoint_every, int bg_from_fields,
+                    const double *const *seed_fld, const double *const *seed_pi, const double seed_bg[2],
+                    double *const *grad_fld, double *const *grad_pi, double *grad_coeff, clmi_background *grad_bg);
+
 /* ---- tuning and timing ------------------------------------------------------------------------ */
 
 /* No counterpart in the reference (tile shapes belong to this library's kernels).  Times every

@@ -452,3 +452,180 @@ def grf_u1(N: int, dx: float, spec, seed: int, stream: int, mass2, aDot: float, 
     j0k = np.fft.rfftn(-dx * dx * rho, axes=(0, 1, 2))
     piA = [np.fft.irfftn(p, s=s, axes=(0, 1, 2)) for p in gauss_solve_modes(j0k, N)]
     return phi, pi, piA
+
+
+# ---------------------------------------------------------------------------------------------
+# Reverse-mode derivative of the leapfrog run (scalar singlets), FP64 numpy
+# ---------------------------------------------------------------------------------------------
+
+def _lap_np(f, dx):
+    return sum(np.roll(f, 1, a) + np.roll(f, -1, a) - 2 * f for a in range(3)) / (dx * dx)
+
+
+def _monomials(spec, phi):
+    """Per term t: value m_t, first derivatives dm_t/dphi_f, second derivatives (lists of arrays)."""
+    nf = spec.n_scalars
+    out = []
+    for _, e in spec.terms:
+        e = list(e) + [0] * (nf - len(e))
+        m = np.ones_like(phi[0])
+        for f in range(nf):
+            m = m * phi[f] ** e[f]
+        d1 = []
+        for n in range(nf):
+            if e[n] == 0:
+                d1.append(np.zeros_like(phi[0]))
+                continue
+            p = np.full_like(phi[0], float(e[n]))
+            for f in range(nf):
+                p = p * phi[f] ** (e[f] - 1 if f == n else e[f])
+            d1.append(p)
+        d2 = [[None] * nf for _ in range(nf)]
+        for n in range(nf):
+            for k in range(nf):
+                if n == k:
+                    if e[n] < 2:
+                        d2[n][k] = np.zeros_like(phi[0])
+                        continue
+                    p = np.full_like(phi[0], float(e[n] * (e[n] - 1)))
+                    for f in range(nf):
+                        p = p * phi[f] ** (e[f] - 2 if f == n else e[f])
+                else:
+                    if e[n] == 0 or e[k] == 0:
+                        d2[n][k] = np.zeros_like(phi[0])
+                        continue
+                    p = np.full_like(phi[0], float(e[n] * e[k]))
+                    for f in range(nf):
+                        p = p * phi[f] ** (e[f] - 1 if f in (n, k) else e[f])
+                d2[n][k] = p
+        out.append((m, d1, d2))
+    return out
+
+
+def _sf_partials(spec, a, pi2, G, U):
+    al, fm = spec.alpha, spec.fStar / spec.MPl
+    A = a ** (2 * al + 1) / 3.0 * fm * fm
+    B = (al - 2) * 0.5 * pi2 * a ** -6 + al * 0.5 * G * a ** -2 + (al + 1) * U
+    return dict(K=A * B, dpi2=A * (al - 2) * 0.5 * a ** -6, dG=A * al * 0.5 * a ** -2, dU=A * (al + 1),
+                da=(2 * al + 1) / a * A * B + A * ((al - 2) * 0.5 * pi2 * -6 * a ** -7 + al * 0.5 * G * -2 * a ** -3))
+
+
+def adjoint_lf(spec, N, dx, dt, phi0, pi0, nsteps, seed_fld, seed_pi, seed_bg=(0.0, 0.0), expansion=True):
+    """FP64 restatement of the reverse-mode leapfrog: forward map = setAllAverages + aDotI from the
+    Friedmann constraint, LeapFrog constructor, nsteps steps, sync (on the C oracle, every state
+    kept); returns (dL/dphi_0, dL/dpi_0, dL/dcoeff) for the given output seeds."""
+    nf = spec.n_scalars
+    V = float(N ** 3)
+    al = spec.alpha
+    orc = OracleLattice(spec, N, dx, dt, expansion=expansion)
+    for f in range(nf):
+        orc.upload(0, f, 0, phi0[f])
+        orc.upload(1, f, 0, pi0[f])
+    if expansion:
+        _, _, rhs = orc.hubble_constraint()
+        orc.set_background(aDotI=math.sqrt(rhs))
+    else:
+        orc.set_all_averages()
+    b_init = orc.background()
+    orc.lf_init()
+    states = [([orc.download(0, f) for f in range(nf)], [orc.download(1, f) for f in range(nf)])]
+    tape = [orc.background()]
+    for _ in range(nsteps):
+        orc.lf_evolve(1)
+        states.append(([orc.download(0, f) for f in range(nf)], [orc.download(1, f) for f in range(nf)]))
+        tape.append(orc.background())
+    orc.lf_sync()
+    b_fin = orc.background()
+    phiT = [orc.download(0, f) for f in range(nf)]
+    piT = [orc.download(1, f) for f in range(nf)]
+    coeff = [c for c, _ in spec.terms]
+
+    lphi = [np.array(s, dtype=np.float64) for s in seed_fld]
+    lpi = [np.array(s, dtype=np.float64) for s in seed_pi]
+    gc = np.zeros(len(coeff))
+
+    def kick_T(phi, c, aK, aV):
+        mons = _monomials(spec, phi)
+        RK = sum(np.sum(lpi[f] * _lap_np(phi[f], dx)) for f in range(nf))
+        RV = 0.0
+        for f in range(nf):
+            h = sum(coeff[t] * sum(mons[t][2][f][g] * lpi[g] for g in range(nf)) for t in range(len(coeff)))
+            lphi[f] += c * (aK * _lap_np(lpi[f], dx) - aV * h)
+        for t in range(len(coeff)):
+            st = sum(np.sum(lpi[f] * mons[t][1][f]) for f in range(nf))
+            gc[t] += -c * aV * st
+            RV += coeff[t] * st
+        return RK, RV
+
+    lam_aI, lam_aDotSI, lam_pi2SI, lam_G, lam_U = seed_bg[0], 0.0, 0.0, 0.0, 0.0
+    bT = tape[nsteps]
+    lam_pi2T = 0.0
+    if expansion:
+        lam_aDotSI = seed_bg[1]
+        lamK = 0.5 * dt * seed_bg[1]
+        d = _sf_partials(spec, bT["aI"], b_fin["pi2AvI"], bT["grad2AvI"], bT["potAvI"])
+        lam_pi2T = lamK * d["dpi2"]
+        lam_G, lam_U = lamK * d["dG"], lamK * d["dU"]
+        lam_aI += lamK * d["da"]
+    for f in range(nf):
+        lpi[f] += lam_pi2T * 2.0 / V * piT[f]
+    aK, aV = bT["aI"] ** (1 + al), bT["aI"] ** (3 + al)
+    RK, RV = kick_T(phiT, 0.5 * dt, aK, aV)
+    if expansion:
+        lam_aI += 0.5 * dt * (RK * (1 + al) * bT["aI"] ** al - RV * (3 + al) * bT["aI"] ** (2 + al))
+
+    for n in range(nsteps - 1, -1, -1):
+        bn, bn1 = tape[n], tape[n + 1]
+        phi1, pih = states[n + 1]
+        mons = _monomials(spec, phi1)
+        for f in range(nf):
+            dv = sum(coeff[t] * mons[t][1][f] for t in range(len(coeff)))
+            lphi[f] += lam_G * (-2.0 / V) * _lap_np(phi1[f], dx) + lam_U * dv / V
+        for t in range(len(coeff)):
+            gc[t] += lam_U * np.sum(mons[t][0]) / V
+        Ra = sum(np.sum(lphi[f] * pih[f]) for f in range(nf))
+        F = bn1["aSI"] ** (al - 3)
+        lamP = 0.0
+        if expansion:
+            lam_aSI1 = dt * Ra * (al - 3) * bn1["aSI"] ** (al - 4)
+            lam_aI1 = lam_aI + 0.5 * lam_aSI1
+            lam_aI_n = 0.5 * lam_aSI1 + lam_aI1
+            lam_aDotSI1 = lam_aDotSI + dt * lam_aI1
+            lam_aDotSI_n = lam_aDotSI1
+            lamK = dt * lam_aDotSI1
+            d = _sf_partials(spec, bn["aI"], bn1["pi2AvI"], bn["grad2AvI"], bn["potAvI"])
+            lam_pi2I = lamK * d["dpi2"]
+            lam_G_n, lam_U_n = lamK * d["dG"], lamK * d["dU"]
+            lam_aI_n += lamK * d["da"]
+            lam_pi2SI_n = 0.5 * lam_pi2I
+            lamP = 0.5 * lam_pi2I + lam_pi2SI
+        for f in range(nf):
+            lpi[f] += dt * F * lphi[f] + lamP * 2.0 / V * pih[f]
+        w = 0.5 if n == 0 else 1.0
+        aK, aV = bn["aI"] ** (1 + al), bn["aI"] ** (3 + al)
+        RK, RV = kick_T(states[n][0], w * dt, aK, aV)
+        if expansion:
+            lam_aI_n += w * dt * (RK * (1 + al) * bn["aI"] ** al - RV * (3 + al) * bn["aI"] ** (2 + al))
+            lam_aI, lam_aDotSI, lam_pi2SI, lam_G, lam_U = lam_aI_n, lam_aDotSI_n, lam_pi2SI_n, lam_G_n, lam_U_n
+
+    if expansion:
+        lam_aDotI0 = lam_aDotSI
+        lamK0 = -0.5 * dt * lam_aDotSI
+        d = _sf_partials(spec, b_init["aI"], b_init["pi2AvI"], b_init["grad2AvI"], b_init["potAvI"])
+        lam_pi2I0 = lamK0 * d["dpi2"]
+        lam_G += lamK0 * d["dG"]
+        lam_U += lamK0 * d["dU"]
+        a0, fm = b_init["aI"], spec.fStar / spec.MPl
+        lamRho = lam_aDotI0 / (2.0 * b_init["aDotI"]) * a0 ** (2 * al + 2) / 3.0 * fm * fm
+        lamP0 = lam_pi2I0 + lam_pi2SI + lamRho * 0.5 * a0 ** -6
+        lamG0 = lam_G + lamRho * 0.5 * a0 ** -2
+        lamU0 = lam_U + lamRho
+        phi_0, pi_0 = states[0]
+        mons = _monomials(spec, phi_0)
+        for f in range(nf):
+            dv = sum(coeff[t] * mons[t][1][f] for t in range(len(coeff)))
+            lphi[f] += lamG0 * (-2.0 / V) * _lap_np(phi_0[f], dx) + lamU0 * dv / V
+            lpi[f] += lamP0 * 2.0 / V * pi_0[f]
+        for t in range(len(coeff)):
+            gc[t] += lamU0 * np.sum(mons[t][0]) / V
+    return lphi, lpi, gc

@@ -541,3 +541,84 @@ def test_u1_ics_statistics_and_structure(oracle_mod):
         assert abs(r2.mean() - 1) < 5 / math.sqrt(r2.size)
         back = np.fft.rfftn(np.fft.irfftn(phik, s=(N, N, N), axes=(0, 1, 2)), axes=(0, 1, 2))
         np.testing.assert_allclose(back, phik, atol=1e-12 * np.abs(phik).max())
+
+
+# ------------------------------------------------------------------------------------------------
+# Reverse-mode leapfrog
+# ------------------------------------------------------------------------------------------------
+
+def _fd_loss(oracle_mod, spec, N, dx, dt, phi0, pi0, nsteps, expansion, weights):
+    """A smooth scalar of the final state: a fixed linear functional of phi_T, pi_T plus a(T)."""
+    o = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=expansion)
+    for f in range(spec.n_scalars):
+        o.upload(FLD_S, f, 0, phi0[f])
+        o.upload(PI_S, f, 0, pi0[f])
+    if expansion:
+        _, _, rhs = o.hubble_constraint()
+        o.set_background(aDotI=math.sqrt(rhs))
+    else:
+        o.set_all_averages()
+    o.lf_init()
+    o.lf_evolve(nsteps)
+    o.lf_sync()
+    wf, wp, wa = weights
+    bg = o.background()
+    return (sum(np.sum(wf[f] * o.download(FLD_S, f) ** 2) + np.sum(wp[f] * o.download(PI_S, f))
+                for f in range(spec.n_scalars)) + wa[0] * bg["aI"] + wa[1] * bg["aDotI"])
+
+
+@pytest.mark.parametrize("expansion", [True, False])
+def test_adjoint_restatement_matches_central_differences(oracle_mod, expansion):
+    """The FP64 reverse-mode leapfrog (including the scale-factor chain and the initial
+    background computed from the fields) against central differences of the forward run, for a
+    loss quadratic in phi_T and linear in pi_T, a(T), aDot(T); directional derivatives along random
+    directions and every potential coefficient."""
+    import dataclasses
+
+    spec = lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+    N, dt, kIR, nsteps = 8, 0.005, 0.8, 12
+    dx = 2 * math.pi / kIR / N
+    rng = np.random.default_rng(21)
+    phi0 = np.stack([spec.fld0[f] + 0.05 * rng.normal(size=(N,) * 3) for f in range(2)])
+    pi0 = np.stack([spec.pi0[f] + 0.05 * rng.normal(size=(N,) * 3) for f in range(2)])
+    wf = [rng.normal(size=(N,) * 3) / N ** 3 for _ in range(2)]
+    wp = [rng.normal(size=(N,) * 3) / N ** 3 for _ in range(2)]
+    wa = (0.7, -1.3)
+    # seeds of the loss at the final state
+    o = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=expansion)
+    for f in range(2):
+        o.upload(FLD_S, f, 0, phi0[f])
+        o.upload(PI_S, f, 0, pi0[f])
+    if expansion:
+        _, _, rhs = o.hubble_constraint()
+        o.set_background(aDotI=math.sqrt(rhs))
+    else:
+        o.set_all_averages()
+    o.lf_init()
+    o.lf_evolve(nsteps)
+    o.lf_sync()
+    seed_f = [2 * wf[f] * o.download(FLD_S, f) for f in range(2)]
+    gphi, gpi, gc = oracle_mod.adjoint_lf(spec, N, dx, dt, phi0, pi0, nsteps, seed_f, wp,
+                                          wa if expansion else (0.0, 0.0), expansion)
+    weights = (wf, wp, wa if expansion else (0.0, 0.0))
+    for which in ("phi", "pi"):
+        d = rng.normal(size=phi0.shape)
+        eps = 1e-5
+        if which == "phi":
+            lp = _fd_loss(oracle_mod, spec, N, dx, dt, phi0 + eps * d, pi0, nsteps, expansion, weights)
+            lm = _fd_loss(oracle_mod, spec, N, dx, dt, phi0 - eps * d, pi0, nsteps, expansion, weights)
+            ad = sum(np.sum(gphi[f] * d[f]) for f in range(2))
+        else:
+            lp = _fd_loss(oracle_mod, spec, N, dx, dt, phi0, pi0 + eps * d, nsteps, expansion, weights)
+            lm = _fd_loss(oracle_mod, spec, N, dx, dt, phi0, pi0 - eps * d, nsteps, expansion, weights)
+            ad = sum(np.sum(gpi[f] * d[f]) for f in range(2))
+        assert ad == pytest.approx((lp - lm) / (2 * eps), rel=1e-6)
+    for t in range(len(spec.terms)):
+        c0 = spec.terms[t][0]
+        h = 1e-5 * abs(c0)
+        vals = []
+        for sgn in (1, -1):
+            terms = [(c + (sgn * h if i == t else 0.0), e) for i, (c, e) in enumerate(spec.terms)]
+            vals.append(_fd_loss(oracle_mod, dataclasses.replace(spec, terms=terms), N, dx, dt, phi0, pi0, nsteps,
+                                 expansion, weights))
+        assert gc[t] == pytest.approx((vals[0] - vals[1]) / (2 * h), rel=1e-6)
