@@ -122,6 +122,65 @@ def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
     }
 
 
+def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
+    """Batched ensemble: 16 realisations each at 64^3, 128^3 and 192^3 (distinct seeds), FP32-bulk,
+    advanced together (one batched launch per step, graph-replayed) versus the same lattices advanced
+    one at a time; plus the ensemble-mean chi spectrum of the 64^3 members against the same
+    realisations in FP64."""
+    from cosmolattice_amd.clmi import FLD_S, Ensemble, power_spectrum
+
+    sizes = [64] * 16 + [128] * 16 + [192] * 16
+    log(f"[bench] ensemble: {len(sizes)} lattices, {steps} steps")
+    members = []
+    for i, N in enumerate(sizes):
+        lat, _ = make_lattice(clmi, spec, N, kIR, dt, clmi.CLMI_F32_BULK, seed + i)
+        members.append(lat)
+    ens = Ensemble(members)
+    ens.lf_evolve(2)  # first step of each member alone, then the graph is captured
+    sites = sum(N ** 3 for N in sizes)
+    t0 = time.perf_counter()
+    ens.lf_evolve(steps)
+    t_ens = time.perf_counter() - t0
+    for m in members:
+        m.synchronize()
+    t0 = time.perf_counter()
+    for m in members:
+        m.lf_evolve(steps)
+    for m in members:
+        m.synchronize()
+    t_one = time.perf_counter() - t0
+    # ensemble-mean chi spectrum of the 64^3 members vs FP64 twins
+    nspec = 16
+    twins = []
+    for i in range(nspec):
+        lat, _ = make_lattice(clmi, spec, 64, kIR, dt, clmi.CLMI_F64, seed + i)
+        lat.lf_evolve(2 + 2 * steps)
+        twins.append(lat)
+    def mean_ps(lats):
+        acc = None
+        for lat in lats:
+            lat.lf_sync()
+            s_, c_, m_ = lat.spectrum_bins(FLD_S, 1, 0, 1.0)
+            _, ps = power_spectrum(s_, c_, m_, 64, kIR, 1.0, 1, 1)
+            acc = ps if acc is None else acc + ps
+        return acc / len(lats)
+    p32, p64 = mean_ps(members[:nspec]), mean_ps(twins)
+    band = np.abs(p32[1:] - p64[1:]) / np.maximum(np.abs(p64[1:]), 1e-300)
+    ens.close()
+    for lat in members + twins:
+        lat.close()
+    return {
+        "config": "48 lphi4 2-field realisations: 16 x 64^3, 16 x 128^3, 16 x 192^3, FP32-bulk, leapfrog",
+        "steps": steps,
+        "site_updates": sites,
+        "ensemble_site_updates_per_s": sites * steps / t_ens,
+        "one_at_a_time_site_updates_per_s": sites * steps / t_one,
+        "speedup": t_one / t_ens,
+        "ms_per_step": 1e3 * t_ens / steps,
+        "mean_chi_spectrum_rel_err_vs_fp64": {"mean": float(band.mean()), "max": float(band.max())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +191,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="clmi_config.flags for both lattices")
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -199,6 +260,9 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle)")
         cpu = cpu_baseline(lats["f64"], spec, N, 2 * math.pi / kIR / N, dt, args.cpu_seconds)
+    for lat in lats.values():
+        lat.close()
+    ens = None if args.no_ensemble else ensemble_bench(clmi, spec, args.ensemble_steps)
 
     line = {
         "metric": METRIC,
@@ -224,6 +288,7 @@ def main():
         "f32_over_f64": r32["site_updates_per_s"] / r64["site_updates_per_s"],
         "hbm_frac_f32": roof(r32)["frac"],
         "accuracy": acc,
+        "ensemble": ens,
     }
     print(json.dumps(line), flush=True)
 

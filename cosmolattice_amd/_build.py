@@ -26,6 +26,7 @@ HIP_SOURCES = [
     "scalar_rows.hip",
     "scalar_rows_f32.hip",
     "scalar_rows_f64.hip",
+    "scalar_rows_batch.hip",
     "gauge_pass.hip",
     "background.hip",
     "fft_kernels.hip",

@@ -118,6 +118,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_adjoint_lf": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_double), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_double), C.POINTER(Background)]),
+        "clmi_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+        "clmi_ensemble_destroy": (None, [P]),
+        "clmi_ensemble_last_error": (C.c_char_p, [P]),
+        "clmi_ensemble_lf_evolve": (C.c_int, [P, C.c_int]),
         "clmi_profile_enable": (C.c_int, [P, C.c_int]),
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
@@ -470,6 +474,39 @@ def power_spectrum(sum_f2, count, mean_n, N: int, kIR: float, dk: float = 1.0, p
     else:
         raise ValueError("ps_type must be 1 or 2")
     return nt * kIR, ps
+
+
+class Ensemble:
+    """Independent lattices (N = 64, 128, 192; same precision, field count and expansion) advanced
+    together: one batched launch per leapfrog step, replayed from a captured HIP graph
+    (clmi_ensemble_*).  The member Lattices stay usable for initial conditions and observations."""
+
+    def __init__(self, members: Sequence[Lattice]):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.handle.value for m in self.members])
+        h = C.c_void_p()
+        rc = self.lib.clmi_ensemble_create(arr, len(self.members), C.byref(h))
+        if rc != CLMI_OK:
+            raise ClmiError(f"clmi_ensemble_create failed ({rc}): {self.lib.clmi_ensemble_last_error(None).decode()}")
+        self._h = h
+
+    def lf_evolve(self, nsteps: int = 1):
+        rc = self.lib.clmi_ensemble_lf_evolve(self._h, nsteps)
+        if rc != CLMI_OK:
+            raise ClmiError(f"clmi_ensemble_lf_evolve failed ({rc}): "
+                            f"{self.lib.clmi_ensemble_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.clmi_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LeapFrog:

@@ -26,6 +26,22 @@ __global__ __launch_bounds__(kUpdThreads) void k_update(DevState *st, const doub
   apply_update(st, red, mc, ops);
 }
 
+// The same update for several lattices, one workgroup each.
+__global__ __launch_bounds__(kUpdThreads) void k_update_batch(const UpdMember *m, UpdateOps ops)
+{
+  const UpdMember &u = m[blockIdx.x];
+  __shared__ double tot[kNumRed];
+  const int t = threadIdx.x;
+  if (u.rows > 0) {
+    sum_partials<kUpdThreads>(u.partials, u.rows, tot);
+    if (t < kNumRed) u.st->red[t] = tot[t] * u.mc.invV;
+  }
+  if (t != 0) return;
+  double red[kNumRed];
+  for (int k = 0; k < kNumRed; ++k) red[k] = u.rows > 0 ? tot[k] * u.mc.invV : u.st->red[k];
+  apply_update(u.st, red, u.mc, ops);
+}
+
 // GaussLaws::checkU1 (definitions/gausslaws.h:31-53) with the charge density of
 // definitions/mattercurrents.h:27-48.  Slots: |LHS-RHS|, |LHS1^2+LHS2^2+LHS3^2+RHS^2|, |RHS|, |LHS|.
 template <bool F32>
@@ -127,6 +143,12 @@ hipError_t launch_update(DevState *st, const double *partials, int rows, const M
                          const UpdateOps &ops, hipStream_t s)
 {
   hipLaunchKernelGGL(k_update, dim3(1), dim3(kUpdThreads), 0, s, st, partials, rows, mc, ops);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_batch(const UpdMember *d_members, int n, const UpdateOps &ops, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_update_batch, dim3(n), dim3(kUpdThreads), 0, s, d_members, ops);
   return hipGetLastError();
 }
 

@@ -1296,6 +1296,195 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   return CLMI_OK;
 }
 
+}  // extern "C"
+
+// ---- ensembles ----------------------------------------------------------------------------------
+
+struct clmi_ensemble {
+  std::vector<clmi_ctx *> m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  RowsMember *d_rows[2] = {nullptr, nullptr};  // member tables for the two buffer parities
+  UpdMember *d_upd = nullptr;
+  int total_blocks = 0;
+  int nf = 0;
+  bool f32 = false, ex = false;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int efail(clmi_ensemble *e, int code, const std::string &msg)
+{
+  if (e)
+    e->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define EHIP(e, expr)                                                                                         \
+  do {                                                                                                        \
+    hipError_t e_ = (expr);                                                                                   \
+    if (e_ != hipSuccess) return efail((e), CLMI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// Member tables for the current buffer parities (A) and the flipped ones (B).
+int ens_tables(clmi_ensemble *e)
+{
+  std::vector<RowsMember> ta(e->m.size()), tb(e->m.size());
+  std::vector<UpdMember> tu(e->m.size());
+  int start = 0;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    clmi_ctx *c = e->m[i];
+    const int fi = c->cur_fS, pi = c->cur_pS;
+    for (int par = 0; par < 2; ++par) {
+      const int f0 = par ? 1 - fi : fi, p0 = par ? 1 - pi : pi;
+      RowsMember r{};
+      r.a = scalar_args(c, f0, 1 - f0, p0, 1 - p0, c->cfg.dt, c->cfg.dt, 0);
+      r.XC = c->rplan.XC;
+      r.G = c->rplan.G;
+      r.blocks = c->rplan.blocks;
+      r.start = start;
+      r.vz = c->N / 64;
+      (par ? tb : ta)[i] = r;
+    }
+    UpdMember u{};
+    u.st = c->dstate;
+    u.partials = c->partials;
+    u.rows = c->rplan.rows;
+    u.mc = c->mc;
+    tu[i] = u;
+    start += (c->rplan.blocks + 7) / 8 * 8;
+  }
+  e->total_blocks = start;
+  EHIP(e, hipMemcpyAsync(e->d_rows[0], ta.data(), sizeof(RowsMember) * ta.size(), hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(e->d_rows[1], tb.data(), sizeof(RowsMember) * tb.size(), hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(e->d_upd, tu.data(), sizeof(UpdMember) * tu.size(), hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  return CLMI_OK;
+}
+
+// One steady-state leapfrog step of every member: fused drift+kick pass, then the background update.
+int ens_step(clmi_ensemble *e, int parity)
+{
+  const int n = (int)e->m.size();
+  EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, true, true, e->ex, e->d_rows[parity], n, e->total_blocks,
+                                   e->stream));
+  if (e->ex) EHIP(e, launch_update_batch(e->d_upd, n, make_ops({{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}),
+                                         e->stream));
+  return CLMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clmi_ensemble_create(clmi_ctx *const *members, int n, clmi_ensemble **out)
+{
+  if (!out) return efail(nullptr, CLMI_E_INVALID, "null output");
+  *out = nullptr;
+  if (!members || n < 1) return efail(nullptr, CLMI_E_INVALID, "no members");
+  const clmi_ctx *c0 = members[0];
+  if (!c0) return efail(nullptr, CLMI_E_INVALID, "null member");
+  for (int i = 0; i < n; ++i) {
+    const clmi_ctx *c = members[i];
+    if (!c) return efail(nullptr, CLMI_E_INVALID, "null member");
+    if (c->device != c0->device || c->f32 != c0->f32 || c->cfg.n_scalars != c0->cfg.n_scalars ||
+        c->cfg.expansion != c0->cfg.expansion)
+      return efail(nullptr, CLMI_E_INVALID, "members must share device, precision, field count and expansion");
+    if (c->cfg.n_cscalars != 0 || !c->rowk || c->N > 192 || c->rplan.WPR != 1 || c->rplan.TY != 8 ||
+        c->rplan.VZ * 64 != c->N)
+      return efail(nullptr, CLMI_E_INVALID,
+                   "ensemble members must be scalar-singlet lattices with N = 64, 128 or 192 (row pass)");
+    for (int j = 0; j < i; ++j)
+      if (members[j] == c) return efail(nullptr, CLMI_E_INVALID, "a context appears twice");
+  }
+  auto *e = new clmi_ensemble();
+  e->m.assign(members, members + n);
+  e->device = c0->device;
+  e->nf = c0->cfg.n_scalars;
+  e->f32 = c0->f32;
+  e->ex = c0->cfg.expansion != 0;
+  int rc = CLMI_OK;
+  if (hipSetDevice(e->device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
+      hipMalloc(&e->d_rows[0], sizeof(RowsMember) * n) != hipSuccess ||
+      hipMalloc(&e->d_rows[1], sizeof(RowsMember) * n) != hipSuccess ||
+      hipMalloc(&e->d_upd, sizeof(UpdMember) * n) != hipSuccess)
+    rc = efail(nullptr, CLMI_E_NOMEM, "ensemble allocation failed");
+  if (rc != CLMI_OK) {
+    clmi_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return CLMI_OK;
+}
+
+void clmi_ensemble_destroy(clmi_ensemble *e)
+{
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (auto *p : e->d_rows)
+    if (p) (void)hipFree(p);
+  if (e->d_upd) (void)hipFree(e->d_upd);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char *clmi_ensemble_last_error(const clmi_ensemble *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int clmi_ensemble_lf_evolve(clmi_ensemble *e, int nsteps)
+{
+  if (!e) return efail(nullptr, CLMI_E_INVALID, "null ensemble");
+  if (nsteps < 0) return efail(e, CLMI_E_INVALID, "nsteps < 0");
+  if (nsteps == 0) return CLMI_OK;
+  EHIP(e, hipSetDevice(e->device));
+  // members that are synced or fully drifted take their first step on their own
+  bool steady = true;
+  for (clmi_ctx *c : e->m) steady = steady && c->pending_drift && !c->synced && !c->prof;
+  if (!steady) {
+    for (clmi_ctx *c : e->m) {
+      if (c->prof) return efail(e, CLMI_E_STATE, "disable profiling on ensemble members");
+      if (clmi_lf_evolve(c, 1) != CLMI_OK) return efail(e, CLMI_E_HIP, std::string("member step: ") + c->err);
+    }
+    --nsteps;
+  }
+  for (clmi_ctx *c : e->m) EHIP(e, hipStreamSynchronize(c->stream));
+  CHK(ens_tables(e));
+  // two steps (both buffer parities) per graph replay
+  if (nsteps >= 2 && !e->exec) {
+    EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
+    int rc = ens_step(e, 0);
+    if (rc == CLMI_OK) rc = ens_step(e, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc != CLMI_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    EHIP(e, ce);
+    e->graph = g;
+    EHIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  }
+  for (int k = 0; k + 1 < nsteps; k += 2) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+  if (nsteps % 2) CHK(ens_step(e, 0));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  for (clmi_ctx *c : e->m) {
+    if (nsteps % 2) {
+      c->cur_fS = 1 - c->cur_fS;
+      c->cur_pS = 1 - c->cur_pS;
+    }
+    c->pending_drift = true;
+    c->synced = false;
+  }
+  return CLMI_OK;
+}
+
 int clmi_profile_enable(clmi_ctx *c, int on)
 {
   CHK(check_ctx(c));

@@ -164,6 +164,26 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
   }
 }
 
+// ---- ensembles: several lattices in one launch (scalar_rows_batch.hip, background.hip) ----
+struct RowsMember {   // one lattice's fused pass inside a batched launch (device-resident table)
+  ScalarPassArgs a;
+  int XC, G;
+  int blocks;         // workgroups of this lattice
+  int start;          // first workgroup index of this lattice in the batched grid (multiple of 8)
+  int vz;             // row shape: VZ = N / 64 (WPR = 1, TY = 8)
+  int pad;
+};
+hipError_t launch_scalar_rows_batch(int nf, bool f32, bool drift, bool kick, bool red, const RowsMember *d_members,
+                                    int n, int total_blocks, hipStream_t s);
+struct UpdMember {    // one lattice's background update inside a batched launch
+  DevState *st;
+  const double *partials;
+  int rows;
+  int pad;
+  ModelConsts mc;
+};
+hipError_t launch_update_batch(const UpdMember *d_members, int n, const UpdateOps &ops, hipStream_t s);
+
 // ---- adjoint (adjoint_kernels.hip) ----
 constexpr int kAdjRed = 16;  // partial-sum slots of the adjoint reductions
 struct AdjFields {           // forward fields of one leapfrog state, in storage precision

@@ -213,19 +213,23 @@ __device__ __forceinline__ void block_reduce_rows(double (&v)[K], double *row)
   }
 }
 
+// LDS tile of one workgroup: plane x in buffer (x - x0) & 1.
+template <int NF, bool F32, int VZ, int WPR, int TY>
+using RowsTile = typename Tr<F32>::V[2][NF][TY + 2][64 * VZ * WPR];
+
+// One workgroup's work: workgroup b of the nwg that cover this lattice.
 template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, bool NT>
-__global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
+__device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC, const int G, const int b,
+                                          const int nwg, RowsTile<NF, F32, VZ, WPR, TY> &ldsbuf)
 {
   using T = Tr<F32>;
   using V = typename T::V;
   using Raw = typename T::template Raw<VZ>;
   constexpr int NW = WPR * TY;  // wavefronts per workgroup
   constexpr int N = 64 * VZ * WPR;
-  __shared__ V ldsbuf[2][NF][TY + 2][N];  // tile of plane x in buffer (x - x0) & 1
 
   const uint32_t NN = (uint32_t)N * N;
   const int nTy = N / TY;
-  const int nwg = gridDim.x, b = blockIdx.x;
   const int L = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
   const int ty = L % nTy, cx = L / nTy;
   const int y0 = ty * TY, x0 = cx * XC;
@@ -428,6 +432,13 @@ __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassA
       __syncthreads();  // LDS tile of plane x+1 ready
     }
   }
+}
+
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, bool NT>
+__global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
+{
+  __shared__ RowsTile<NF, F32, VZ, WPR, TY> ldsbuf;
+  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, NT>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
 }
 
 template <int NF, bool F32, int VZ, int WPR, int TY, bool NT>

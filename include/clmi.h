@@ -205,6 +205,22 @@ int clmi_grf_scalar(clmi_ctx *ctx, int index, uint64_t seed, uint32_t stream, do
 int clmi_grf_u1(clmi_ctx *ctx, uint64_t seed, uint32_t stream, const double mass2[2], double aDot, double kIR,
                 double kCutOff, const double fld0[2], const double pi0[2]);
 
+/* ---- ensembles --------------------------------------------------------------------------------- */
+
+/* No counterpart in the reference (which runs one realisation per process).  A group of independent
+ * lattices -- same device, precision, number of scalar singlets and expansion setting; N = 64, 128 or
+ * 192 each; potentials, dt, dx and initial conditions free -- advanced together: every leapfrog step
+ * of every member is one batched lattice launch plus one batched background launch, replayed from a
+ * captured HIP graph.  Each member's fields and background are bit-identical to clmi_lf_evolve on
+ * that member alone.  Members keep their own contexts for initial conditions, observations and
+ * sync; the ensemble borrows them and must be destroyed before any member. */
+typedef struct clmi_ensemble clmi_ensemble;
+int clmi_ensemble_create(clmi_ctx *const *members, int n, clmi_ensemble **out);
+void clmi_ensemble_destroy(clmi_ensemble *ens);
+const char *clmi_ensemble_last_error(const clmi_ensemble *ens);
+/* nsteps x LeapFrog::evolve on every member (evolvers/leapfrog.h:42-89), as clmi_lf_evolve. */
+int clmi_ensemble_lf_evolve(clmi_ensemble *ens, int nsteps);
+
 /* ---- adjoint ----------------------------------------------------------------------------------- */
 
 /* No counterpart in the reference (a reverse-mode derivative of its leapfrog).  Runs, from the state

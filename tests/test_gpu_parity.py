@@ -446,3 +446,69 @@ def test_grf_u1_parity(gpu, oracle_mod):
         say("gauss", prec, g)
         assert g[0] < (1e-13 if prec == CLMI_F64 else 1e-6)
         lat.close()
+
+
+@pytest.mark.parametrize("prec", [CLMI_F32_BULK, CLMI_F64])
+def test_ensemble_bit_identical_to_individual_runs(gpu, prec):
+    """An ensemble of uneven lattices (64^3, 128^3, 192^3; different seeds, couplings and dt)
+    advanced in batched, graph-replayed launches equals each lattice evolved on its own, bit for
+    bit, after every call (graph replays interleaved with observations and syncs)."""
+    from cosmolattice_amd.clmi import Ensemble
+
+    say(f"case ensemble prec={prec}")
+    kIR = 0.2
+    members, twins = [], []
+    for i, (N, q, dt) in enumerate([(64, 100.0, 0.005), (128, 80.0, 0.004), (192, 100.0, 0.005), (64, 120.0, 0.006)]):
+        spec = bench_spec(q=q)
+        dx = 2 * math.pi / kIR / N
+        pair = []
+        for _ in range(2):
+            lat = Lattice(spec, N, dx, dt, prec, expansion=True)
+            lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=0.5)
+            m2 = spec.mass2(spec.fld0)
+            for f in range(2):
+                lat.grf_scalar(f, 1000 + i, f, m2[f], 0.5, kIR, N * kIR / 4, spec.fld0[f], spec.pi0[f])
+            _, _, rhs = lat.hubble_constraint()
+            lat.set_background(aDotI=math.sqrt(rhs))
+            lat.lf_init()
+            pair.append(lat)
+        members.append(pair[0])
+        twins.append(pair[1])
+    ens = Ensemble(members)
+
+    def compare(tag):
+        for m, t in zip(members, twins):
+            for f in range(2):
+                for k in (FLD_S, PI_S):
+                    assert np.array_equal(m.download(k, f, 0), t.download(k, f, 0)), (tag, m.N, k, f)
+            assert m.background() == t.background(), (tag, m.N)
+
+    for chunk in (1, 5, 4, 3):
+        ens.lf_evolve(chunk)
+        for t in twins:
+            t.lf_evolve(chunk)
+        compare(f"after {chunk}")
+    for lat in members + twins:
+        lat.lf_sync()
+    compare("sync")
+    ens.lf_evolve(6)
+    for t in twins:
+        t.lf_evolve(6)
+    compare("after sync")
+    ens.close()
+    for lat in members + twins:
+        lat.close()
+
+
+def test_ensemble_rejects_unsupported_members(gpu):
+    from cosmolattice_amd.clmi import ClmiError, Ensemble
+
+    spec = bench_spec()
+    a = Lattice(spec, 64, 0.5, 0.005, CLMI_F32_BULK)
+    b = Lattice(spec, 64, 0.5, 0.005, CLMI_F64)
+    c = Lattice(spec, 256, 0.5, 0.005, CLMI_F32_BULK)
+    for group in ([a, b], [a, c], [a, a]):
+        with pytest.raises(ClmiError):
+            Ensemble(group)
+    for lat in (a, b, c):
+        lat.close()
