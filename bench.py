@@ -135,12 +135,17 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
     for i, N in enumerate(sizes):
         lat, _ = make_lattice(clmi, spec, N, kIR, dt, clmi.CLMI_F32_BULK, seed + i)
         members.append(lat)
-    ens = Ensemble(members)
-    ens.lf_evolve(2)  # first step of each member alone, then the graph is captured
     sites = sum(N ** 3 for N in sizes)
-    t0 = time.perf_counter()
-    ens.lf_evolve(steps)
-    t_ens = time.perf_counter() - t0
+    # all members of a size in one group (one launch per size per step) and cache-sized groups
+    timings = {}
+    for label, gb in (("one_group_per_size", 1 << 40), ("cache_groups", 0)):
+        ens = Ensemble(members, group_bytes=gb)
+        ens.lf_evolve(2)  # first step of each member alone, then the graphs are captured
+        t0 = time.perf_counter()
+        ens.lf_evolve(steps)
+        timings[label] = (time.perf_counter() - t0, ens.groups)
+        ens.close()
+    t_ens, ngroups = timings["cache_groups"]
     for m in members:
         m.synchronize()
     t0 = time.perf_counter()
@@ -154,7 +159,7 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
     twins = []
     for i in range(nspec):
         lat, _ = make_lattice(clmi, spec, 64, kIR, dt, clmi.CLMI_F64, seed + i)
-        lat.lf_evolve(2 + 2 * steps)
+        lat.lf_evolve(4 + 3 * steps)
         twins.append(lat)
     def mean_ps(lats):
         acc = None
@@ -166,7 +171,6 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
         return acc / len(lats)
     p32, p64 = mean_ps(members[:nspec]), mean_ps(twins)
     band = np.abs(p32[1:] - p64[1:]) / np.maximum(np.abs(p64[1:]), 1e-300)
-    ens.close()
     for lat in members + twins:
         lat.close()
     return {
@@ -174,6 +178,8 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
         "steps": steps,
         "site_updates": sites,
         "ensemble_site_updates_per_s": sites * steps / t_ens,
+        "groups": ngroups,
+        "one_group_per_size_site_updates_per_s": sites * steps / timings["one_group_per_size"][0],
         "one_at_a_time_site_updates_per_s": sites * steps / t_one,
         "speedup": t_one / t_ens,
         "ms_per_step": 1e3 * t_ens / steps,

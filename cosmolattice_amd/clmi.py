@@ -118,7 +118,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_adjoint_lf": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_double), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_double), C.POINTER(Background)]),
-        "clmi_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+        "clmi_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
+        "clmi_ensemble_groups": (C.c_int, [P]),
         "clmi_ensemble_destroy": (None, [P]),
         "clmi_ensemble_last_error": (C.c_char_p, [P]),
         "clmi_ensemble_lf_evolve": (C.c_int, [P, C.c_int]),
@@ -481,12 +482,12 @@ class Ensemble:
     together: one batched launch per leapfrog step, replayed from a captured HIP graph
     (clmi_ensemble_*).  The member Lattices stay usable for initial conditions and observations."""
 
-    def __init__(self, members: Sequence[Lattice]):
+    def __init__(self, members: Sequence[Lattice], group_bytes: int = 0):
         self.lib = load_library()
         self.members = list(members)
         arr = (C.c_void_p * len(self.members))(*[m.handle.value for m in self.members])
         h = C.c_void_p()
-        rc = self.lib.clmi_ensemble_create(arr, len(self.members), C.byref(h))
+        rc = self.lib.clmi_ensemble_create(arr, len(self.members), group_bytes, C.byref(h))
         if rc != CLMI_OK:
             raise ClmiError(f"clmi_ensemble_create failed ({rc}): {self.lib.clmi_ensemble_last_error(None).decode()}")
         self._h = h
@@ -496,6 +497,10 @@ class Ensemble:
         if rc != CLMI_OK:
             raise ClmiError(f"clmi_ensemble_lf_evolve failed ({rc}): "
                             f"{self.lib.clmi_ensemble_last_error(self._h).decode()}")
+
+    @property
+    def groups(self) -> int:
+        return self.lib.clmi_ensemble_groups(self._h)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1301,16 +1301,24 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
 // ---- ensembles ----------------------------------------------------------------------------------
 
 struct clmi_ensemble {
-  std::vector<clmi_ctx *> m;
+  std::vector<clmi_ctx *> m;      // as given
+  std::vector<clmi_ctx *> order;  // sorted by size class (row shape VZ = N / 64)
   int device = 0;
   hipStream_t stream = nullptr;
-  RowsMember *d_rows[2] = {nullptr, nullptr};  // member tables for the two buffer parities
+  RowsMember *d_rows[2] = {nullptr, nullptr};  // member tables (order) for the two buffer parities
   UpdMember *d_upd = nullptr;
-  int total_blocks = 0;
+  size_t group_bytes = 0;
+  // A group: consecutive members of one size class whose evolved arrays fit the byte budget; it is
+  // advanced for all requested steps before the next group, so its arrays stay in the last-level
+  // cache when they fit.
+  struct Group {
+    int vz, first, count, blocks;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Group> groups;
   int nf = 0;
   bool f32 = false, ex = false;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
   std::string err;
 };
 
@@ -1331,50 +1339,79 @@ int efail(clmi_ensemble *e, int code, const std::string &msg)
     if (e_ != hipSuccess) return efail((e), CLMI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
   } while (0)
 
-// Member tables for the current buffer parities (A) and the flipped ones (B).
+size_t state_bytes(const clmi_ctx *c) { return fused_pass_bytes(c) / 2; }
+
+// Groups (formed once) and member tables for the current buffer parities (A) and the flipped ones
+// (B).  Workgroup starts restart at 0 in every group.
 int ens_tables(clmi_ensemble *e)
 {
-  std::vector<RowsMember> ta(e->m.size()), tb(e->m.size());
-  std::vector<UpdMember> tu(e->m.size());
-  int start = 0;
-  for (size_t i = 0; i < e->m.size(); ++i) {
-    clmi_ctx *c = e->m[i];
-    const int fi = c->cur_fS, pi = c->cur_pS;
-    for (int par = 0; par < 2; ++par) {
-      const int f0 = par ? 1 - fi : fi, p0 = par ? 1 - pi : pi;
-      RowsMember r{};
-      r.a = scalar_args(c, f0, 1 - f0, p0, 1 - p0, c->cfg.dt, c->cfg.dt, 0);
-      r.XC = c->rplan.XC;
-      r.G = c->rplan.G;
-      r.blocks = c->rplan.blocks;
-      r.start = start;
-      r.vz = c->N / 64;
-      (par ? tb : ta)[i] = r;
+  const size_t n = e->order.size();
+  if (e->groups.empty()) {
+    size_t bytes = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const clmi_ctx *c = e->order[i];
+      const int vz = c->N / 64;
+      const size_t b = state_bytes(c);
+      if (e->groups.empty() || e->groups.back().vz != vz || bytes + b > e->group_bytes) {
+        clmi_ensemble::Group g{};
+        g.vz = vz;
+        g.first = (int)i;
+        e->groups.push_back(g);
+        bytes = 0;
+      }
+      bytes += b;
+      e->groups.back().count++;
     }
-    UpdMember u{};
-    u.st = c->dstate;
-    u.partials = c->partials;
-    u.rows = c->rplan.rows;
-    u.mc = c->mc;
-    tu[i] = u;
-    start += (c->rplan.blocks + 7) / 8 * 8;
   }
-  e->total_blocks = start;
-  EHIP(e, hipMemcpyAsync(e->d_rows[0], ta.data(), sizeof(RowsMember) * ta.size(), hipMemcpyHostToDevice, e->stream));
-  EHIP(e, hipMemcpyAsync(e->d_rows[1], tb.data(), sizeof(RowsMember) * tb.size(), hipMemcpyHostToDevice, e->stream));
-  EHIP(e, hipMemcpyAsync(e->d_upd, tu.data(), sizeof(UpdMember) * tu.size(), hipMemcpyHostToDevice, e->stream));
+  std::vector<RowsMember> ta(n), tb(n);
+  std::vector<UpdMember> tu(n);
+  for (auto &g : e->groups) {
+    int start = 0;
+    for (int i = g.first; i < g.first + g.count; ++i) {
+      clmi_ctx *c = e->order[i];
+      const int fi = c->cur_fS, pi = c->cur_pS;
+      for (int par = 0; par < 2; ++par) {
+        const int f0 = par ? 1 - fi : fi, p0 = par ? 1 - pi : pi;
+        RowsMember r{};
+        r.a = scalar_args(c, f0, 1 - f0, p0, 1 - p0, c->cfg.dt, c->cfg.dt, 0);
+        r.XC = c->rplan.XC;
+        r.G = c->rplan.G;
+        r.blocks = c->rplan.blocks;
+        r.start = start;
+        r.vz = g.vz;
+        (par ? tb : ta)[i] = r;
+      }
+      UpdMember u{};
+      u.st = c->dstate;
+      u.partials = c->partials;
+      u.rows = c->rplan.rows;
+      u.mc = c->mc;
+      tu[i] = u;
+      start += (c->rplan.blocks + 7) / 8 * 8;
+    }
+    if (g.exec && g.blocks != start) {  // a member's launch plan changed since the graph was captured
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+      g.graph = nullptr;
+    }
+    g.blocks = start;
+  }
+  EHIP(e, hipMemcpyAsync(e->d_rows[0], ta.data(), sizeof(RowsMember) * n, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(e->d_rows[1], tb.data(), sizeof(RowsMember) * n, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, hipMemcpyAsync(e->d_upd, tu.data(), sizeof(UpdMember) * n, hipMemcpyHostToDevice, e->stream));
   EHIP(e, hipStreamSynchronize(e->stream));
   return CLMI_OK;
 }
 
-// One steady-state leapfrog step of every member: fused drift+kick pass, then the background update.
-int ens_step(clmi_ensemble *e, int parity)
+// One steady-state leapfrog step of a group: fused drift+kick pass, then the background update.
+int ens_step(clmi_ensemble *e, const clmi_ensemble::Group &g, int parity)
 {
-  const int n = (int)e->m.size();
-  EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, true, true, e->ex, e->d_rows[parity], n, e->total_blocks,
-                                   e->stream));
-  if (e->ex) EHIP(e, launch_update_batch(e->d_upd, n, make_ops({{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}),
-                                         e->stream));
+  EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, g.vz, true, true, e->ex, e->d_rows[parity] + g.first, g.count,
+                                   g.blocks, e->stream));
+  if (e->ex)
+    EHIP(e, launch_update_batch(e->d_upd + g.first, g.count,
+                                make_ops({{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}), e->stream));
   return CLMI_OK;
 }
 
@@ -1382,7 +1419,7 @@ int ens_step(clmi_ensemble *e, int parity)
 
 extern "C" {
 
-int clmi_ensemble_create(clmi_ctx *const *members, int n, clmi_ensemble **out)
+int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, clmi_ensemble **out)
 {
   if (!out) return efail(nullptr, CLMI_E_INVALID, "null output");
   *out = nullptr;
@@ -1404,10 +1441,13 @@ int clmi_ensemble_create(clmi_ctx *const *members, int n, clmi_ensemble **out)
   }
   auto *e = new clmi_ensemble();
   e->m.assign(members, members + n);
+  e->order = e->m;
+  std::stable_sort(e->order.begin(), e->order.end(), [](const clmi_ctx *a, const clmi_ctx *b) { return a->N < b->N; });
   e->device = c0->device;
   e->nf = c0->cfg.n_scalars;
   e->f32 = c0->f32;
   e->ex = c0->cfg.expansion != 0;
+  e->group_bytes = group_bytes ? group_bytes : ((size_t)96 << 20);
   int rc = CLMI_OK;
   if (hipSetDevice(e->device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
       hipMalloc(&e->d_rows[0], sizeof(RowsMember) * n) != hipSuccess ||
@@ -1427,8 +1467,10 @@ void clmi_ensemble_destroy(clmi_ensemble *e)
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (auto &g : e->groups) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
   for (auto *p : e->d_rows)
     if (p) (void)hipFree(p);
   if (e->d_upd) (void)hipFree(e->d_upd);
@@ -1437,6 +1479,8 @@ void clmi_ensemble_destroy(clmi_ensemble *e)
 }
 
 const char *clmi_ensemble_last_error(const clmi_ensemble *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int clmi_ensemble_groups(const clmi_ensemble *e) { return e ? (int)e->groups.size() : 0; }
 
 int clmi_ensemble_lf_evolve(clmi_ensemble *e, int nsteps)
 {
@@ -1456,23 +1500,25 @@ int clmi_ensemble_lf_evolve(clmi_ensemble *e, int nsteps)
   }
   for (clmi_ctx *c : e->m) EHIP(e, hipStreamSynchronize(c->stream));
   CHK(ens_tables(e));
-  // two steps (both buffer parities) per graph replay
-  if (nsteps >= 2 && !e->exec) {
-    EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
-    int rc = ens_step(e, 0);
-    if (rc == CLMI_OK) rc = ens_step(e, 1);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-    if (rc != CLMI_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
+  for (auto &g : e->groups) {
+    // two steps (both buffer parities) per graph replay
+    if (nsteps >= 2 && !g.exec) {
+      EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
+      int rc = ens_step(e, g, 0);
+      if (rc == CLMI_OK) rc = ens_step(e, g, 1);
+      hipGraph_t gr = nullptr;
+      const hipError_t ce = hipStreamEndCapture(e->stream, &gr);
+      if (rc != CLMI_OK) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      EHIP(e, ce);
+      g.graph = gr;
+      EHIP(e, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
     }
-    EHIP(e, ce);
-    e->graph = g;
-    EHIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    for (int k = 0; k + 1 < nsteps; k += 2) EHIP(e, hipGraphLaunch(g.exec, e->stream));
+    if (nsteps % 2) CHK(ens_step(e, g, 0));
   }
-  for (int k = 0; k + 1 < nsteps; k += 2) EHIP(e, hipGraphLaunch(e->exec, e->stream));
-  if (nsteps % 2) CHK(ens_step(e, 0));
   EHIP(e, hipStreamSynchronize(e->stream));
   for (clmi_ctx *c : e->m) {
     if (nsteps % 2) {

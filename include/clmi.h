@@ -207,15 +207,20 @@ int clmi_grf_u1(clmi_ctx *ctx, uint64_t seed, uint32_t stream, const double mass
 
 /* ---- ensembles --------------------------------------------------------------------------------- */
 
-/* No counterpart in the reference (which runs one realisation per process).  A group of independent
- * lattices -- same device, precision, number of scalar singlets and expansion setting; N = 64, 128 or
- * 192 each; potentials, dt, dx and initial conditions free -- advanced together: every leapfrog step
- * of every member is one batched lattice launch plus one batched background launch, replayed from a
- * captured HIP graph.  Each member's fields and background are bit-identical to clmi_lf_evolve on
- * that member alone.  Members keep their own contexts for initial conditions, observations and
- * sync; the ensemble borrows them and must be destroyed before any member. */
+/* No counterpart in the reference (which runs one realisation per process).  Independent lattices --
+ * same device, precision, number of scalar singlets and expansion setting; N = 64, 128 or 192 each;
+ * potentials, dt, dx and initial conditions free -- advanced together.  Members of one size are
+ * packed into groups whose evolved arrays total at most group_bytes (0: 96 MiB, about what the
+ * last-level cache holds for reading and writing); every leapfrog step of a group is one batched
+ * lattice launch plus one batched background launch, replayed from a captured HIP graph, and each
+ * group runs all requested steps before the next starts.  Each member's fields and background are
+ * bit-identical to clmi_lf_evolve on that member alone.  Members keep their own contexts for initial
+ * conditions, observations and sync; the ensemble borrows them and must be destroyed before any
+ * member. */
 typedef struct clmi_ensemble clmi_ensemble;
-int clmi_ensemble_create(clmi_ctx *const *members, int n, clmi_ensemble **out);
+int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, clmi_ensemble **out);
+/* Number of member groups (formed at the first clmi_ensemble_lf_evolve; 0 before). */
+int clmi_ensemble_groups(const clmi_ensemble *ens);
 void clmi_ensemble_destroy(clmi_ensemble *ens);
 const char *clmi_ensemble_last_error(const clmi_ensemble *ens);
 /* nsteps x LeapFrog::evolve on every member (evolvers/leapfrog.h:42-89), as clmi_lf_evolve. */

@@ -448,14 +448,15 @@ def test_grf_u1_parity(gpu, oracle_mod):
         lat.close()
 
 
-@pytest.mark.parametrize("prec", [CLMI_F32_BULK, CLMI_F64])
-def test_ensemble_bit_identical_to_individual_runs(gpu, prec):
+@pytest.mark.parametrize("prec,group_bytes", [(CLMI_F32_BULK, 0), (CLMI_F64, 0), (CLMI_F32_BULK, 1),
+                                               (CLMI_F32_BULK, 1 << 40)])
+def test_ensemble_bit_identical_to_individual_runs(gpu, prec, group_bytes):
     """An ensemble of uneven lattices (64^3, 128^3, 192^3; different seeds, couplings and dt)
     advanced in batched, graph-replayed launches equals each lattice evolved on its own, bit for
     bit, after every call (graph replays interleaved with observations and syncs)."""
     from cosmolattice_amd.clmi import Ensemble
 
-    say(f"case ensemble prec={prec}")
+    say(f"case ensemble prec={prec} group_bytes={group_bytes}")
     kIR = 0.2
     members, twins = [], []
     for i, (N, q, dt) in enumerate([(64, 100.0, 0.005), (128, 80.0, 0.004), (192, 100.0, 0.005), (64, 120.0, 0.006)]):
@@ -474,7 +475,7 @@ def test_ensemble_bit_identical_to_individual_runs(gpu, prec):
             pair.append(lat)
         members.append(pair[0])
         twins.append(pair[1])
-    ens = Ensemble(members)
+    ens = Ensemble(members, group_bytes=group_bytes)
 
     def compare(tag):
         for m, t in zip(members, twins):
