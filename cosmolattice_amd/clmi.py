@@ -108,6 +108,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_hubble_constraint": (C.c_int, [P, C.POINTER(C.c_double)]),
         "clmi_gauss_u1": (C.c_int, [P, C.POINTER(C.c_double)]),
         "clmi_field_means": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+        "clmi_field_stats": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "clmi_monomial_averages": (C.c_int, [P, C.POINTER(C.c_double)]),
         "clmi_spectrum_nbins": (C.c_int, [P, C.c_double]),
         "clmi_spectrum_bins": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -389,6 +391,16 @@ class Lattice:
         out = (C.c_double * 2)()
         self._check(self.lib.clmi_field_means(self._h, kind, index, comp, scale, out), "field_means")
         return tuple(out)
+
+    def field_stats(self, kind: int, index: int, comp: int = 0) -> tuple:
+        out = (C.c_double * 3)()
+        self._check(self.lib.clmi_field_stats(self._h, kind, index, comp, out), "field_stats")
+        return tuple(out)
+
+    def monomial_averages(self) -> list:
+        out = (C.c_double * max(1, len(self.spec.terms)))()
+        self._check(self.lib.clmi_monomial_averages(self._h, out), "monomial_averages")
+        return list(out)[:len(self.spec.terms)]
 
     def spectrum_bins(self, kind: int, index: int, comp: int = 0, dk: float = 1.0):
         nb = self.lib.clmi_spectrum_nbins(self._h, dk)

@@ -84,19 +84,47 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const GaussArgs a)
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void k_means(const MeansArgs a)
 {
-  double acc[2] = {0, 0};
+  double acc[3] = {0, 0, 0};
+  auto ld = [&](uint32_t i) -> double {
+    if (F32) return (double)((const float *)a.p)[i] + (a.lo ? (double)a.lo[i] : 0.0);
+    return ((const double *)a.p)[i];
+  };
+  const uint32_t N = a.N, NN = N * N;
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
-    double v;
-    if (F32)
-      v = (double)((const float *)a.p)[s] + (a.lo ? (double)a.lo[s] : 0.0);
-    else
-      v = ((const double *)a.p)[s];
-    v *= a.scale;
+    const double c = ld(s);
+    const double v = c * a.scale;
     acc[0] += v;
     acc[1] += v * v;
+    if (a.grad) {
+      const uint32_t x = s / NN, y = (s - x * NN) / N, z = s - x * NN - y * N;
+      const double dx = ld(s - x * NN + (x + 1 == N ? 0 : x + 1) * NN) - c;
+      const double dy = ld(s - y * N + (y + 1 == N ? 0 : y + 1) * N) - c;
+      const double dz = ld(s - z + (z + 1 == N ? 0 : z + 1)) - c;
+      acc[2] += dx * dx + dy * dy + dz * dz;
+    }
   }
-  double v[kNumRed] = {acc[0], acc[1], 0, 0, 0, 0, 0, 0};
+  double v[kNumRed] = {acc[0], acc[1], acc[2], 0, 0, 0, 0, 0};
   block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_monomials(const MonoArgs a)
+{
+  double acc[kNumRed];
+#pragma unroll
+  for (int k = 0; k < kNumRed; ++k) acc[k] = 0.0;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
+    double phi[CLMI_MAX_SCALARS];
+    for (int f = 0; f < a.nf; ++f)
+      phi[f] = F32 ? (double)((const float *)a.phi[f])[s] + (double)a.philo[f][s] : ((const double *)a.phi[f])[s];
+    for (int t = 0; t < a.poly.nterms && t < kNumRed; ++t) {
+      double m = 1.0;
+      for (int f = 0; f < a.nf; ++f)
+        for (int k = 0; k < a.poly.e[t][f]; ++k) m *= phi[f];
+      acc[t] += m;
+    }
+  }
+  block_reduce_store<kNumRed>(acc, a.partials + (size_t)blockIdx.x * kNumRed);
 }
 
 __global__ __launch_bounds__(kUpdThreads) void k_sum_rows(const double *partials, int rows, double *out)
@@ -143,6 +171,15 @@ hipError_t launch_update(DevState *st, const double *partials, int rows, const M
                          const UpdateOps &ops, hipStream_t s)
 {
   hipLaunchKernelGGL(k_update, dim3(1), dim3(kUpdThreads), 0, s, st, partials, rows, mc, ops);
+  return hipGetLastError();
+}
+
+hipError_t launch_monomials(const MonoArgs &a, int blocks, hipStream_t s)
+{
+  if (a.f32)
+    hipLaunchKernelGGL(k_monomials<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_monomials<false>, dim3(blocks), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

@@ -813,6 +813,8 @@ int clmi_field_means(clmi_ctx *c, int kind, int index, int comp, double scale, d
   a.f32 = c->f32;
   a.scale = scale;
   a.partials = c->partials;
+  a.N = (uint32_t)c->N;
+  a.grad = 0;
   const int blocks = scalar_pass_blocks(c->V);
   HIPCHK(c, launch_means(a, blocks, c->stream));
   HIPCHK(c, launch_sum_rows(c->partials, blocks, c->dscratch, c->stream));
@@ -821,6 +823,62 @@ int clmi_field_means(clmi_ctx *c, int kind, int index, int comp, double scale, d
   HIPCHK(c, hipStreamSynchronize(c->stream));
   out[0] = h[0] / (double)c->V;
   out[1] = h[1] / (double)c->V;
+  return CLMI_OK;
+}
+
+int clmi_field_stats(clmi_ctx *c, int kind, int index, int comp, double out[3])
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  CHK(flush(c));
+  DevArr *arr = nullptr;
+  bool is_field = false;
+  CHK(select_arr(c, kind, index, comp, &arr, &is_field));
+  MeansArgs a{};
+  a.p = arr->p;
+  a.lo = arr->lo;
+  a.V = c->V;
+  a.f32 = c->f32;
+  a.scale = 1.0;
+  a.partials = c->partials;
+  a.N = (uint32_t)c->N;
+  a.grad = 1;
+  const int blocks = scalar_pass_blocks(c->V);
+  HIPCHK(c, launch_means(a, blocks, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, blocks, c->dscratch, c->stream));
+  double h[kNumRed];
+  HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double dx = c->cfg.dx;
+  out[0] = h[0] / (double)c->V;
+  out[1] = h[1] / (double)c->V;
+  out[2] = h[2] / (dx * dx) / (double)c->V;
+  return CLMI_OK;
+}
+
+int clmi_monomial_averages(clmi_ctx *c, double *out)
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  if (c->cfg.n_scalars == 0) return fail(c, CLMI_E_INVALID, "no scalar singlets");
+  CHK(flush(c));
+  MonoArgs a{};
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    a.phi[f] = c->fS[f][c->cur_fS].p;
+    a.philo[f] = c->fS[f][c->cur_fS].lo;
+  }
+  a.nf = c->cfg.n_scalars;
+  a.f32 = c->f32;
+  a.V = c->V;
+  a.poly = c->spoly;
+  a.partials = c->partials;
+  const int blocks = scalar_pass_blocks(c->V);
+  HIPCHK(c, launch_monomials(a, blocks, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, blocks, c->dscratch, c->stream));
+  double h[kNumRed];
+  HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < c->spoly.nterms; ++t) out[t] = h[t] / (double)c->V;
   return CLMI_OK;
 }
 

@@ -258,7 +258,18 @@ struct MeansArgs {
   int f32;
   double scale;
   double *partials;
+  uint32_t N;   // with grad: slot 2 = sum over sites of sum_i (f(x+i) - f(x))^2 (unscaled differences)
+  int grad;
 };
+struct MonoArgs {  // slot t = sum over sites of monomial t of the scalar potential
+  const void *phi[CLMI_MAX_SCALARS];
+  const float *philo[CLMI_MAX_SCALARS];
+  int nf, f32;
+  uint32_t V;
+  ScalarPoly poly;
+  double *partials;
+};
+hipError_t launch_monomials(const MonoArgs &a, int blocks, hipStream_t s);
 hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s);
 hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s);
 

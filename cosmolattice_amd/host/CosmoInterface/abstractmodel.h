@@ -288,6 +288,11 @@ public:
   }
   bool expansionOn() const { return expansion_; }
 
+  // Per potential term k: (index into the library's scalar monomial list, coefficient) of each of its
+  // monomials, so that <potentialTerms(Tag<k>)> = sum coefficient * <monomial>.  Filled when the
+  // context is created; empty for terms in |phi| of the complex scalar.
+  const std::vector<std::vector<std::pair<int, double>>> &termMonomials() const { return termMonomials_; }
+
 private:
   template <size_t... K> Poly sumTerms(std::index_sequence<K...>)
   {
@@ -331,6 +336,22 @@ private:
       throw PotentialDerivativeNotDefined("potDeriv(Tag<" + std::to_string(J) + ">) is not defined");
     }
   }
+  template <size_t... K> void collectTerms(const clmi_config &cfg, std::index_sequence<K...>)
+  {
+    R &self = static_cast<R &>(*this);
+    termMonomials_.clear();
+    auto one = [&](const Poly &term) {
+      std::vector<std::pair<int, double>> list;
+      for (const auto &[m, c] : term.terms())
+        for (int t = 0; t < cfg.n_terms; ++t) {
+          bool same = m[kNormCSVar] == 0;
+          for (int v = 0; v < CLMI_MAX_SCALARS; ++v) same = same && m[v] == cfg.term_exp[t][v];
+          if (same) list.push_back({t, c});
+        }
+      termMonomials_.push_back(list);
+    };
+    (one(Poly(self.potentialTerms(Tag<(int)K>{}))), ...);
+  }
   void fillPotential(clmi_config &cfg)
   {
     const Poly V = potentialPolynomial();
@@ -352,10 +373,12 @@ private:
         ++cfg.n_terms;
       }
     }
+    collectTerms(cfg, std::make_index_sequence<NPotTerms>{});
   }
 
   clmi_ctx *ctx_ = nullptr;
   bool expansion_ = true;
+  std::vector<std::vector<std::pair<int, double>>> termMonomials_;
 };
 
 }  // namespace TempLat

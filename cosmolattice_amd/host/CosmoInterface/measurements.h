@@ -62,6 +62,31 @@ template <class Model> EnergyTotals energyTotals(Model &model, const clmi_averag
   return e;
 }
 
+// Power spectrum of one real component with the normalisations of PowerSpectrumMeasurer
+// (measurements/powerspectrum.h:85-110): returns (k = nominal bin centre x kIR, PS) per bin.
+template <class Model>
+void powerSpectrum(Model &model, int kind, int index, int comp, double dk, int psType, int psVersion,
+                   std::vector<double> &k, std::vector<double> &ps)
+{
+  clmi_ctx *ctx = model.context();
+  const int nb = clmi_spectrum_nbins(ctx, dk);
+  if (nb < 1) throw std::runtime_error("bad spectrum bin width");
+  std::vector<double> s(nb), cnt(nb), mn(nb);
+  model.check(clmi_spectrum_bins(ctx, kind, index, comp, dk, nb, s.data(), cnt.data(), mn.data()),
+              "clmi_spectrum_bins");
+  const double N = (double)model.N, kIR = model.kIR, dx = model.dx, pi = 3.14159265358979323846;
+  k.assign(nb, 0.0);
+  ps.assign(nb, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    const double nt = psVersion == 1 ? mn[b] : b * dk;
+    k[b] = b * dk * kIR;
+    if (psType == 1)
+      ps[b] = kIR * nt * dx / dk / std::pow(N, 5) / 2 / pi * s[b];
+    else
+      ps[b] = std::pow(kIR * nt * dx / N, 3) / dk / 2 / (pi * pi) * (cnt[b] > 0 ? s[b] / cnt[b] : 0.0);
+  }
+}
+
 template <class Model> class Measurer
 {
 public:
@@ -71,14 +96,21 @@ public:
       : outputFreq((int)std::lround(par.tOutFreq / par.dt)),
         infreqOutputFreq((int)std::lround(par.tOutInfreq / par.dt)),
         rareOutputFreq((int)std::lround(par.tOutRareFreq / par.dt)), expansion(par.expansion),
+        printHeaders(par.printHeaders), dk(par.deltaKBin), psType(par.powerSpectrumType),
+        psVersion(par.powerSpectrumVersion), dir(par.outFn),
         energyCons(par.outFn + "average_energy_conservation.txt",
                    expansion ? std::vector<std::string>{"t", "rel_diff_friedmann", "LHS_friedmann", "RHS_friedmann"}
                              : std::vector<std::string>{"t", "energy_cons"},
                    par.printHeaders),
-        scaleFactor(par.outFn + "average_scale_factor.txt", {"t", "a", "aDot", "H"}, par.printHeaders)
+        scaleFactor(par.outFn + "average_scale_factor.txt", {"t", "a", "aDot", "H"}, par.printHeaders),
+        energies(par.outFn + "average_energies.txt", energyHeaders(), par.printHeaders)
   {
     if (outputFreq < 1 || infreqOutputFreq < 1 || rareOutputFreq < 1)
       throw RunParametersInconsistent("output intervals must be at least one time step");
+    for (size_t i = 0; i < Model::Ns; ++i)
+      means.emplace_back(par.outFn + "average_scalar_" + std::to_string(i) + ".txt",
+                         std::vector<std::string>{"t", "<phi>", "<pi>", "<phi^2>", "<pi^2>", "rms(phi)", "rms(pi)"},
+                         par.printHeaders);
   }
 
   // measurer.h:181-197
@@ -87,20 +119,91 @@ public:
     return n % outputFreq == 0 || n % infreqOutputFreq == 0 || n % rareOutputFreq == 0;
   }
 
-  // Frequent output: energy conservation (energiesmeasurer.h:105-122) and the scale factor.
   void measure(int n, T t, Model &model)
   {
-    if (n % outputFreq != 0) return;
+    if (n % outputFreq == 0) measureStandard(t, model);
+    if (n % infreqOutputFreq == 0) measureSpectra(t, model);
+  }
+
+  int outputFreq, infreqOutputFreq, rareOutputFreq;
+
+private:
+  static std::vector<std::string> energyHeaders()
+  {
+    std::vector<std::string> h{"t"};
+    for (size_t i = 0; i < Model::Ns; ++i) {
+      h.push_back("E^kin_scal" + std::to_string(i));
+      h.push_back("E^grad_scal" + std::to_string(i));
+    }
+    if (Model::NCs > 0) {
+      h.push_back("E^kin_cmplxscal0");
+      h.push_back("E^grad_cmplxscal0");
+    }
+    if (Model::NU1 > 0) {
+      h.push_back("E^kin_U10");
+      h.push_back("E^grad_U10");
+    }
+    for (size_t k = 0; k < Model::NPotTerms; ++k) h.push_back("Vpot_term_" + std::to_string(k));
+    h.push_back("E_tot");
+    return h;
+  }
+
+  // Frequent output: scalar means (meansmeasurer.h:28-48, scalarsingletmeasurer.h:60-66), energies
+  // (energiesmeasurer.h:45-103), energy conservation (energiesmeasurer.h:105-122), scale factor.
+  void measureStandard(T t, Model &model)
+  {
+    clmi_ctx *ctx = model.context();
+    model.pullBackground();
+    const double a = model.aI, al = model.alpha;
+    std::vector<double> row{(double)t};
+    double etot = 0;
+    for (size_t i = 0; i < Model::Ns; ++i) {
+      double sf[3], sp[3];
+      model.check(clmi_field_stats(ctx, CLMI_FLD_S, (int)i, 0, sf), "clmi_field_stats");
+      model.check(clmi_field_stats(ctx, CLMI_PI_S, (int)i, 0, sp), "clmi_field_stats");
+      const double sc = std::pow(a, al - 3);  // momenta as field velocities
+      const double p1 = sp[0] * sc, p2 = sp[1] * sc * sc;
+      const double rf = sf[1] - sf[0] * sf[0], rp = p2 - p1 * p1;
+      means[i].save({(double)t, sf[0], p1, sf[1], p2, rf > 0 ? std::sqrt(rf) : 0.0, rp > 0 ? std::sqrt(rp) : 0.0});
+      const double ek = 0.5 * sp[1] * std::pow(a, -6.0), eg = 0.5 * sf[2] * std::pow(a, -2.0);
+      row.push_back(ek);
+      row.push_back(eg);
+      etot += ek + eg;
+    }
+    clmi_averages av{};
+    model.check(clmi_volume_averages(ctx, &av), "clmi_volume_averages");
+    const EnergyTotals e = energyTotals(model, av);
+    if (Model::NCs > 0) {
+      row.push_back(e.kinCS);
+      row.push_back(e.gradCS);
+      etot += e.kinCS + e.gradCS;
+    }
+    if (Model::NU1 > 0) {
+      row.push_back(e.elec);
+      row.push_back(e.mag);
+      etot += e.elec + e.mag;
+    }
+    std::vector<double> mono(CLMI_MAX_TERMS, 0.0);
+    if (Model::Ns > 0) model.check(clmi_monomial_averages(ctx, mono.data()), "clmi_monomial_averages");
+    const auto &tm = model.termMonomials();
+    for (size_t k = 0; k < Model::NPotTerms; ++k) {
+      double v = 0;
+      if (Model::NCs > 0 && Model::NPotTerms == 1)
+        v = av.pot;
+      else
+        for (const auto &[idx, c] : tm[k]) v += c * mono[idx];
+      row.push_back(v);
+      etot += v;
+    }
+    row.push_back(etot);
+    energies.save(row);
     if (expansion) {
       double hc[3];
-      model.check(clmi_hubble_constraint(model.context(), hc), "clmi_hubble_constraint");
+      model.check(clmi_hubble_constraint(ctx, hc), "clmi_hubble_constraint");
       model.pullBackground();
       energyCons.save({(double)t, hc[0], hc[1], hc[2]});
       scaleFactor.save({(double)t, model.aI, model.aDotI, model.aDotI / model.aI});
     } else {
-      clmi_averages av{};
-      model.check(clmi_volume_averages(model.context(), &av), "clmi_volume_averages");
-      const double etot = energyTotals(model, av).total;
       if (first_) {
         etot0_ = etot;
         first_ = false;
@@ -109,12 +212,34 @@ public:
     }
   }
 
-  int outputFreq, infreqOutputFreq, rareOutputFreq;
+  // Infrequent output: spectra of every scalar and its momentum (scalarsingletmeasurer.h:74-83),
+  // k, PS(phi), a^(2 alpha - 6) PS(pi) per bin, one block per time.
+  void measureSpectra(T t, Model &model)
+  {
+    const double a = model.aI, al = model.alpha;
+    for (size_t i = 0; i < Model::Ns; ++i) {
+      std::vector<double> k, pf, pp;
+      powerSpectrum(model, CLMI_FLD_S, (int)i, 0, dk, psType, psVersion, k, pf);
+      powerSpectrum(model, CLMI_PI_S, (int)i, 0, dk, psType, psVersion, k, pp);
+      const std::string fn = dir + "spectra_scalar_" + std::to_string(i) + ".txt";
+      std::ofstream out(fn, spectraStarted_ ? std::ios_base::app : std::ios_base::out);
+      if (!out) throw std::runtime_error("cannot open " + fn);
+      out << std::setprecision(15);
+      if (!spectraStarted_ && printHeaders) out << "#k    PS(phi)    PS(pi)    t = " << t << "\n";
+      for (size_t b = 0; b < k.size(); ++b)
+        out << k[b] << "    " << pf[b] << "    " << std::pow(a, 2 * al - 6) * pp[b] << "\n";
+      out << "\n";
+    }
+    spectraStarted_ = true;
+  }
 
-private:
-  bool expansion;
-  AverageFile energyCons, scaleFactor;
-  bool first_ = true;
+  bool expansion, printHeaders;
+  double dk;
+  int psType, psVersion;
+  std::string dir;
+  AverageFile energyCons, scaleFactor, energies;
+  std::vector<AverageFile> means;
+  bool first_ = true, spectraStarted_ = false;
   double etot0_ = 0.0;
 };
 

@@ -171,6 +171,15 @@ int clmi_gauss_u1(clmi_ctx *ctx, double out[3]);
  * array multiplied by scale: out = {<f>, <f^2>}. */
 int clmi_field_means(clmi_ctx *ctx, int kind, int index, int comp, double scale, double out[2]);
 
+/* Volume statistics of one real component: out = {<f>, <f^2>, <sum_i ((f(x+i) - f(x))/dx)^2>}, the
+ * inputs of MeansMeasurer (measurements/meansmeasurer.h:28-48) and of the per-field gradient energy
+ * (definitions/fieldfunctionals.h:52-67, energies.h:37-45). */
+int clmi_field_stats(clmi_ctx *ctx, int kind, int index, int comp, double out[3]);
+/* Volume average of every scalar-potential monomial prod_f phi_f^term_exp[t][f] (without its
+ * coefficient), t = 0..n_terms-1: the potential terms of EnergiesMeasurer (energiesmeasurer.h:85-90)
+ * are sums of these with the model's coefficients. */
+int clmi_monomial_averages(clmi_ctx *ctx, double *out);
+
 /* ---- spectra and initial conditions ---------------------------------------------------------- */
 
 /* Number of radial bins used by clmi_spectrum_bins for bin width dk: floor(sqrt(3)/2*N/dk) + 1

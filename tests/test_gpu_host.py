@@ -61,7 +61,30 @@ def run_program(model, tmp_path, params, *args):
     return r
 
 
-def oracle_loop(orc, steps, out_every, evolver="LF"):
+def observables(orc, spec, N, dx):
+    """Per-field means and energies of the oracle's current state (numpy), in the layout of
+    average_scalar_<i>.txt and average_energies.txt."""
+    bg = orc.background()
+    a, al = bg["aI"], spec.alpha
+    means, erow = [], []
+    for f in range(spec.n_scalars):
+        phi, pi = orc.download(FLD_S, f), orc.download(PI_S, f) * a ** (al - 3)
+        means.append([phi.mean(), pi.mean(), (phi ** 2).mean(), (pi ** 2).mean(),
+                      math.sqrt(max((phi ** 2).mean() - phi.mean() ** 2, 0)),
+                      math.sqrt(max((pi ** 2).mean() - pi.mean() ** 2, 0))])
+        g2 = sum((np.roll(phi, -1, d) - phi) ** 2 for d in range(3)).mean() / dx ** 2
+        erow += [0.5 * (orc.download(PI_S, f) ** 2).mean() * a ** -6, 0.5 * g2 * a ** -2]
+    phis = [orc.download(FLD_S, f) for f in range(spec.n_scalars)]
+    for c, e in spec.terms:
+        m = np.ones_like(phis[0])
+        for f, ef in enumerate(e):
+            m = m * phis[f] ** ef
+        erow.append(c * m.mean())
+    erow.append(sum(erow))
+    return means, erow
+
+
+def oracle_loop(orc, steps, out_every, evolver="LF", obs=None):
     recs = []
     if evolver == "LF":
         orc.lf_init()
@@ -69,6 +92,8 @@ def oracle_loop(orc, steps, out_every, evolver="LF"):
         if i % out_every == 0:
             if evolver == "LF":
                 orc.lf_sync()
+            if obs is not None:
+                obs.append(observables(orc, orc.spec, orc.N, orc.dx))
             recs.append(orc.hubble_constraint())
         if evolver == "LF":
             orc.lf_evolve(1)
@@ -92,7 +117,8 @@ def test_host_program_matches_oracle(gpu, oracle_mod, tmp_path, evolver):
     write_state(tmp_path / "in", {(k, i, 0): a for (k, i), a in ics.items()}, orc.background())
     run_program("lphi4", tmp_path, lphi4_params(N, dx, dt, steps, every, evolver),
                 f"clmi_load_dir={tmp_path / 'in'}", f"clmi_dump_dir={tmp_path}")
-    recs = oracle_loop(orc, steps, every, evolver)
+    obs = []
+    recs = oracle_loop(orc, steps, every, evolver, obs)
     keys = [(k, i, 0) for (k, i) in ics]
     got, bg = read_state(tmp_path, keys, N)
     for (k, i, c) in keys:
@@ -111,6 +137,14 @@ def test_host_program_matches_oracle(gpu, oracle_mod, tmp_path, evolver):
     np.testing.assert_allclose(cons[:, 1], recs[:, 0], rtol=1e-6, atol=1e-13)
     assert (tmp_path / "average_energy_conservation.txt").read_text().startswith(
         "#t    rel_diff_friedmann    LHS_friedmann    RHS_friedmann\n")
+    # scalar means and energies at every output time
+    en = np.loadtxt(tmp_path / "average_energies.txt", ndmin=2)
+    np.testing.assert_allclose(en[:, 1:], np.array([o[1] for o in obs]), rtol=FP64_TOL, atol=1e-14)
+    assert (tmp_path / "average_energies.txt").read_text().startswith(
+        "#t    E^kin_scal0    E^grad_scal0    E^kin_scal1    E^grad_scal1    Vpot_term_0    Vpot_term_1    E_tot\n")
+    for f in range(2):
+        me = np.loadtxt(tmp_path / f"average_scalar_{f}.txt", ndmin=2)
+        np.testing.assert_allclose(me[:, 1:], np.array([o[0][f] for o in obs]), rtol=1e-9, atol=1e-13)
 
 
 def fnv1a64(s: str) -> int:
