@@ -122,6 +122,32 @@ def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
     }
 
 
+def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000):
+    """Charged scalar + U(1) (lphi4U1, e^2/lambda = 10, program units lambda = 1) on 256^3 with the
+    Gauss-consistent vacuum initial conditions: site-updates/s and HBM fraction of the fused pass in
+    both precision modes, and the Gauss-constraint violation at the end."""
+    from cosmolattice_amd.clmi import lphi4U1
+
+    spec = lphi4U1(10.0, phi0_gev=5.6964e18, lam=1.0, pi0_gev2=0.0)
+    dx = 2 * math.pi / kIR / N
+    out = {"config": f"lphi4U1 {N}^3 leapfrog, e^2/lambda = 10, kIR = {kIR}, dt = {dt}, no expansion",
+           "steps": steps}
+    for name, prec in (("f64", clmi.CLMI_F64), ("f32_bulk", clmi.CLMI_F32_BULK)):
+        lat = clmi.Lattice(spec, N, dx, dt, prec, expansion=False)
+        lat.grf_u1(seed, 1, spec.cs_mass2(spec.fld0), 0.0, kIR, N * kIR / 4, spec.fld0, spec.pi0)
+        lat.set_all_averages()
+        lat.lf_init()
+        sec, kms, launches, bpl = timed_run(lat, 10, steps)
+        avg_s = kms / 1e3 / max(launches, 1)
+        g = lat.gauss_u1()
+        out[name] = {"site_updates_per_s": N ** 3 * steps / sec, "ms_per_step": 1e3 * sec / steps,
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": bpl,
+                     "hbm_frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS, "gauss_violation": g[0]}
+        log(f"[bench] u1 {name}: {out[name]['ms_per_step']:.4f} ms/step, frac {out[name]['hbm_frac']:.3f}")
+        lat.close()
+    return out
+
+
 def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
     """Batched ensemble: 16 realisations each at 64^3, 128^3 and 192^3 (distinct seeds), FP32-bulk,
     advanced together (one batched launch per step, graph-replayed) versus the same lattices advanced
@@ -199,6 +225,8 @@ def main():
     ap.add_argument("--flags", type=int, default=0, help="clmi_config.flags for both lattices")
     ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--u1-steps", type=int, default=500)
+    ap.add_argument("--no-u1", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -269,6 +297,7 @@ def main():
     for lat in lats.values():
         lat.close()
     ens = None if args.no_ensemble else ensemble_bench(clmi, spec, args.ensemble_steps)
+    u1 = None if args.no_u1 else u1_bench(clmi, args.u1_steps)
 
     line = {
         "metric": METRIC,
@@ -295,6 +324,7 @@ def main():
         "hbm_frac_f32": roof(r32)["frac"],
         "accuracy": acc,
         "ensemble": ens,
+        "u1": u1,
     }
     print(json.dumps(line), flush=True)
 
