@@ -294,8 +294,6 @@ int flush(clmi_ctx *c)
 {
   if (!c->pending_drift) return CLMI_OK;
   const bool ex = c->cfg.expansion != 0;
-  int rows = 0;
-  (void)rows;
   CHK(pass_update(c, true, false, ex, c->cfg.dt, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
   c->pending_drift = false;
   return CLMI_OK;
@@ -627,13 +625,11 @@ int clmi_lf_evolve(clmi_ctx *c, int nsteps)
   const double dt = c->cfg.dt;
   for (int n = 0; n < nsteps; ++n) {
     const double w = c->synced ? 0.5 : 1.0;
-    int rows = 0;
     if (c->pending_drift) {
       CHK(pass_update(c, true, true, ex, dt, w * dt, w * dt, ex, {{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}));
     } else {
       CHK(pass_update(c, false, true, ex, 0.0, w * dt, w * dt, ex, {{OP_LF_MOMENTA, 0.0}}));
     }
-    (void)rows;
     c->pending_drift = true;
     c->synced = false;
   }
@@ -646,8 +642,6 @@ int clmi_lf_sync(clmi_ctx *c)
   CHK(flush(c));
   if (!c->synced) {
     const bool ex = c->cfg.expansion != 0;
-    int rows = 0;
-    (void)rows;
     CHK(pass_update(c, false, true, ex, 0.0, 0.5 * c->cfg.dt, 0.5 * c->cfg.dt, ex, {{OP_LF_SYNC, 0.0}}));
   }
   c->synced = true;
@@ -667,9 +661,7 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
   for (int n = 0; n < nsteps; ++n)
     for (int s = 0; s < stages; ++s) {
       const double w = ws[s < nws ? s : stages - s - 1];
-      int rows = 0;
       if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
-      (void)rows;
       CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex,
                       {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
       CHK(pass_update(c, true, false, ex, dt * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
@@ -700,8 +692,6 @@ int clmi_set_all_averages(clmi_ctx *c)
 {
   CHK(check_ctx(c));
   CHK(flush(c));
-  int rows = 0;
-  (void)rows;
   CHK(pass_update(c, false, false, true, 0.0, 0.0, 0.0, true, {{OP_SET_ALL, 0.0}}));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CLMI_OK;
@@ -712,8 +702,6 @@ int clmi_volume_averages(clmi_ctx *c, clmi_averages *out)
   CHK(check_ctx(c));
   if (!out) return fail(c, CLMI_E_INVALID, "null output");
   CHK(flush(c));
-  int rows = 0;
-  (void)rows;
   CHK(pass_update(c, false, false, true, 0.0, 0.0, 0.0, true, {}));
   DevState h{};
   CHK(read_state(c, &h));
