@@ -108,6 +108,12 @@ int fail(clmi_ctx *c, int code, const std::string &msg)
 
 size_t bytes_field(bool f32) { return f32 ? 4 : 8; }
 
+// Row-pass shape selector from clmi_config.flags (bit 0: VZ = 4 rows, bit 1: 4-row tiles).
+int rows_shape(int flags)
+{
+  return ((flags & CLMI_FLAG_SHAPE_ALT) ? 1 : 0) | ((flags & CLMI_FLAG_SHAPE_TY4) ? 2 : 0);
+}
+
 int alloc_arr(clmi_ctx *c, DevArr &a, bool field)
 {
   const size_t n = c->V;
@@ -438,8 +444,7 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
     RowsPlan rp;
     for (int xc : {32, 16, 8, 4})
-      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, true,
-                           (g.flags & CLMI_FLAG_SHAPE_ALT) != 0, &rp)) {
+      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, true, rows_shape(g.flags), &rp)) {
         c->rowk = true;
         c->rplan = rp;
         break;
@@ -1610,8 +1615,7 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
     for (int xc : {4, 8, 16, 32, 64}) {
       TiledPlan p;
       RowsPlan rp;
-      if (c->rowk ? !scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, nt != 0,
-                                      (c->cfg.flags & CLMI_FLAG_SHAPE_ALT) != 0, &rp)
+      if (c->rowk ? !scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, nt != 0, rows_shape(c->cfg.flags), &rp)
                   : !scalar_tiled_plan(c->N, c->cfg.n_scalars, xc, nt != 0, &p))
         continue;
       auto launch = [&]() {

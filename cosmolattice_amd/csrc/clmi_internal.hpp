@@ -222,7 +222,7 @@ struct RowsPlan {
   bool NT = false;
   int blocks = 0, rows = 0;
 };
-bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, bool alt, RowsPlan *p);
+bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, int shape, RowsPlan *p);
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,

@@ -11,12 +11,13 @@ hipError_t launch_scalar_rows_f64(int nf, bool drift, bool kick, bool red, const
 // Row-kernel shapes by (N, precision): VZ sites per lane, WPR wavefronts per row, TY rows per
 // workgroup.  The shape fixes the summation order; the x-chunk XC and store mode are tuned at run
 // time without changing results.
-bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, bool alt, RowsPlan *p)
+bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, int shape, RowsPlan *p)
 {
   if (nf != 2 && nf != 1) return false;
   int vz = 0, wpr = 0, ty = 0;
-  if (N == 256 && !alt) { vz = 2; wpr = 2; ty = 8; }
-  else if (N == 256) { vz = 4; wpr = 1; ty = 8; }
+  const bool alt = (shape & 1) != 0, ty4 = (shape & 2) != 0 && nf == 2;
+  if (N == 256 && !alt) { vz = 2; wpr = 2; ty = ty4 ? 4 : 8; }
+  else if (N == 256) { vz = 4; wpr = 1; ty = ty4 ? 4 : 8; }
   else if (N == 128) { vz = 2; wpr = 1; ty = 8; }
   else if (N == 64) { vz = 1; wpr = 1; ty = 8; }
   else if (N == 192) { vz = 3; wpr = 1; ty = 8; }

@@ -16,6 +16,8 @@ hipError_t launch_scalar_rows_f32(int nf, bool drift, bool kick, bool red, const
   CLMI_SHAPE(2, true, 2, 1, 8)
   CLMI_SHAPE(2, true, 1, 1, 8)
   CLMI_SHAPE(2, true, 3, 1, 8)
+  CLMI_SHAPE(2, true, 2, 2, 4)
+  CLMI_SHAPE(2, true, 4, 1, 4)
   CLMI_SHAPE(1, true, 4, 1, 8)
   CLMI_SHAPE(1, true, 2, 2, 8)
   CLMI_SHAPE(1, true, 2, 1, 8)

@@ -16,6 +16,8 @@ hipError_t launch_scalar_rows_f64(int nf, bool drift, bool kick, bool red, const
   CLMI_SHAPE(2, false, 2, 1, 8)
   CLMI_SHAPE(2, false, 1, 1, 8)
   CLMI_SHAPE(2, false, 3, 1, 8)
+  CLMI_SHAPE(2, false, 2, 2, 4)
+  CLMI_SHAPE(2, false, 4, 1, 4)
   CLMI_SHAPE(1, false, 4, 1, 8)
   CLMI_SHAPE(1, false, 2, 2, 8)
   CLMI_SHAPE(1, false, 2, 1, 8)

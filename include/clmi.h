@@ -46,6 +46,9 @@ extern "C" {
  * are bit-identical to the default shape; volume sums differ in summation order only.  Used to
  * compare shapes. */
 #define CLMI_FLAG_SHAPE_ALT 2
+/* clmi_config.flags: 4-row workgroup tiles of the full-row lattice pass (N = 256, two fields),
+ * combinable with CLMI_FLAG_SHAPE_ALT.  Same results as above. */
+#define CLMI_FLAG_SHAPE_TY4 4
 
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
 #define CLMI_FLD_S 0

@@ -114,6 +114,7 @@ def test_lf_lphi4_fp64_parity_100_steps(gpu, oracle_mod, N):
 @pytest.mark.parametrize("N,prec,flags", [
     (64, CLMI_F64, 0), (64, CLMI_F32_BULK, 0), (128, CLMI_F32_BULK, 0), (128, CLMI_F64, 0),
     (192, CLMI_F32_BULK, 0), (192, CLMI_F64, 0), (256, CLMI_F32_BULK, 0), (256, CLMI_F64, 0),
+    (256, CLMI_F32_BULK, 4), (256, CLMI_F64, 6),
     (256, CLMI_F32_BULK, 2), (256, CLMI_F64, 2), (320, CLMI_F32_BULK, 0), (320, CLMI_F64, 0)])
 def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
     """The full-row pass (N = 64, 128, 192, 256, both workgroup shapes) and the tiled x-marching
