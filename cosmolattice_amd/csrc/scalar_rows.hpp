@@ -92,19 +92,27 @@ __device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
   lo = e;
 }
 
+// Lattice arrays live in device (global) memory.  Accesses go through global-address-space
+// pointers even when the pointer itself was read from memory (the batched pass's member table), so
+// the compiler emits global rather than flat instructions; flat loads would share the wait counter
+// of the LDS tile reads and serialise them behind the prefetch.
+template <typename S> using gptr = __attribute__((address_space(1))) S *;
+template <typename S> using cgptr = const __attribute__((address_space(1))) S *;
+
 // VZ consecutive elements with vector accesses (the site index of element 0 is a multiple of VZ).
 template <int VZ, typename S>
-__device__ __forceinline__ void vload(const S *p, uint32_t s, S (&o)[VZ])
+__device__ __forceinline__ void vload(const S *pg, uint32_t s, S (&o)[VZ])
 {
+  const cgptr<S> p = (cgptr<S>)pg;
   if constexpr (VZ == 4 && sizeof(S) == 4) {
     typedef S v4 __attribute__((ext_vector_type(4)));
-    const v4 v = *(const v4 *)(p + s);
+    const v4 v = *(cgptr<v4>)(p + s);
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
   } else if constexpr (VZ % 2 == 0) {
     typedef S v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int k = 0; k < VZ; k += 2) {
-      const v2 v = *(const v2 *)(p + s + k);
+      const v2 v = *(cgptr<v2>)(p + s + k);
       o[k] = v.x;
       o[k + 1] = v.y;
     }
@@ -114,13 +122,14 @@ __device__ __forceinline__ void vload(const S *p, uint32_t s, S (&o)[VZ])
   }
 }
 template <int VZ, bool NT, typename S>
-__device__ __forceinline__ void vstore(S *p, uint32_t s, const S (&o)[VZ])
+__device__ __forceinline__ void vstore(S *pg, uint32_t s, const S (&o)[VZ])
 {
+  const gptr<S> p = (gptr<S>)pg;
   if constexpr (VZ == 4 && sizeof(S) == 4) {
     typedef S v4 __attribute__((ext_vector_type(4)));
     v4 v;
     v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
-    if constexpr (NT) __builtin_nontemporal_store(v, (v4 *)(p + s)); else *(v4 *)(p + s) = v;
+    if constexpr (NT) __builtin_nontemporal_store(v, (gptr<v4>)(p + s)); else *(gptr<v4>)(p + s) = v;
   } else if constexpr (VZ % 2 == 0) {
     typedef S v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
@@ -128,7 +137,7 @@ __device__ __forceinline__ void vstore(S *p, uint32_t s, const S (&o)[VZ])
       v2 v;
       v.x = o[k];
       v.y = o[k + 1];
-      if constexpr (NT) __builtin_nontemporal_store(v, (v2 *)(p + s + k)); else *(v2 *)(p + s + k) = v;
+      if constexpr (NT) __builtin_nontemporal_store(v, (gptr<v2>)(p + s + k)); else *(gptr<v2>)(p + s + k) = v;
     }
   } else {
 #pragma unroll

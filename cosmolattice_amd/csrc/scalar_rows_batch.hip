@@ -13,12 +13,14 @@ namespace {
 constexpr int kBatchThreads = 512;  // 64 * WPR * TY for the shapes (VZ, 1, 8)
 
 template <int NF, bool F32, int VZ, bool D, bool K, bool R>
-__global__ __launch_bounds__(kBatchThreads) void k_scalar_rows_batch(const RowsMember *m, int n)
+__global__ __launch_bounds__(kBatchThreads) void k_scalar_rows_batch(const RowsMember *__restrict__ m, int n)
 {
   __shared__ RowsTile<NF, F32, VZ, 1, 8> lds;
   const int b = blockIdx.x;
   int i = 0;
   while (i + 1 < n && m[i + 1].start <= b) ++i;
+  // the table is read-only for the whole launch (restrict): its loads stay scalar and are not
+  // repeated after the pass's stores
   const RowsMember &mm = m[i];
   const int lb = b - mm.start;
   if (lb >= mm.blocks) return;
