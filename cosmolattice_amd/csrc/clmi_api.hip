@@ -1359,15 +1359,19 @@ struct clmi_ensemble {
   RowsMember *d_rows[2] = {nullptr, nullptr};  // member tables (order) for the two buffer parities
   UpdMember *d_upd = nullptr;
   size_t group_bytes = 0;
-  // A group: consecutive members of one size class whose evolved arrays fit the byte budget; it is
-  // advanced for all requested steps before the next group, so its arrays stay in the last-level
-  // cache when they fit.
+  // A group: consecutive members of one size class (whose evolved arrays fit the byte budget, when
+  // one is set).  Each group has its own stream; a captured graph holds two steps of every group as
+  // parallel branches, so the size classes share the GPU.
   struct Group {
     int vz, first, count, blocks;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
   };
   std::vector<Group> groups;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> captured_blocks;
   int nf = 0;
   bool f32 = false, ex = false;
   std::string err;
@@ -1440,14 +1444,25 @@ int ens_tables(clmi_ensemble *e)
       tu[i] = u;
       start += (c->rplan.blocks + 7) / 8 * 8;
     }
-    if (g.exec && g.blocks != start) {  // a member's launch plan changed since the graph was captured
-      (void)hipGraphExecDestroy(g.exec);
-      (void)hipGraphDestroy(g.graph);
-      g.exec = nullptr;
-      g.graph = nullptr;
-    }
     g.blocks = start;
   }
+  std::vector<int> blocks;
+  for (const auto &g : e->groups) blocks.push_back(g.blocks);
+  if (e->exec && blocks != e->captured_blocks) {  // a member's launch plan changed since the capture
+    (void)hipGraphExecDestroy(e->exec);
+    (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+  }
+  while (e->side.size() < e->groups.size()) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    EHIP(e, hipStreamCreate(&st));
+    e->side.push_back(st);
+    EHIP(e, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    e->join.push_back(ev);
+  }
+  if (!e->fork) EHIP(e, hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
   EHIP(e, hipMemcpyAsync(e->d_rows[0], ta.data(), sizeof(RowsMember) * n, hipMemcpyHostToDevice, e->stream));
   EHIP(e, hipMemcpyAsync(e->d_rows[1], tb.data(), sizeof(RowsMember) * n, hipMemcpyHostToDevice, e->stream));
   EHIP(e, hipMemcpyAsync(e->d_upd, tu.data(), sizeof(UpdMember) * n, hipMemcpyHostToDevice, e->stream));
@@ -1456,13 +1471,41 @@ int ens_tables(clmi_ensemble *e)
 }
 
 // One steady-state leapfrog step of a group: fused drift+kick pass, then the background update.
-int ens_step(clmi_ensemble *e, const clmi_ensemble::Group &g, int parity)
+int ens_step(clmi_ensemble *e, const clmi_ensemble::Group &g, int parity, hipStream_t st)
 {
   EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, g.vz, true, true, e->ex, e->d_rows[parity] + g.first, g.count,
-                                   g.blocks, e->stream));
+                                   g.blocks, st));
   if (e->ex)
     EHIP(e, launch_update_batch(e->d_upd + g.first, g.count,
-                                make_ops({{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}), e->stream));
+                                make_ops({{OP_STORE_FIELDS, 0.0}, {OP_LF_MOMENTA, 0.0}}), st));
+  return CLMI_OK;
+}
+
+// Capture two steps of every group, each group on its own stream (fork/join on the main stream).
+int ens_capture(clmi_ensemble *e)
+{
+  EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
+  int rc = CLMI_OK;
+  hipError_t he = hipEventRecord(e->fork, e->stream);
+  for (size_t i = 0; i < e->groups.size() && rc == CLMI_OK && he == hipSuccess; ++i) {
+    he = hipStreamWaitEvent(e->side[i], e->fork, 0);
+    if (he == hipSuccess) rc = ens_step(e, e->groups[i], 0, e->side[i]);
+    if (rc == CLMI_OK && he == hipSuccess) rc = ens_step(e, e->groups[i], 1, e->side[i]);
+    if (rc == CLMI_OK && he == hipSuccess) he = hipEventRecord(e->join[i], e->side[i]);
+    if (rc == CLMI_OK && he == hipSuccess) he = hipStreamWaitEvent(e->stream, e->join[i], 0);
+  }
+  hipGraph_t gr = nullptr;
+  const hipError_t ce = hipStreamEndCapture(e->stream, &gr);
+  if (rc != CLMI_OK || he != hipSuccess || ce != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc != CLMI_OK) return rc;
+    return efail(e, CLMI_E_HIP, std::string("ensemble graph capture: ") +
+                                     hipGetErrorString(he != hipSuccess ? he : ce));
+  }
+  e->graph = gr;
+  EHIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  e->captured_blocks.clear();
+  for (const auto &g : e->groups) e->captured_blocks.push_back(g.blocks);
   return CLMI_OK;
 }
 
@@ -1498,7 +1541,7 @@ int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, cl
   e->nf = c0->cfg.n_scalars;
   e->f32 = c0->f32;
   e->ex = c0->cfg.expansion != 0;
-  e->group_bytes = group_bytes ? group_bytes : ((size_t)96 << 20);
+  e->group_bytes = group_bytes ? group_bytes : SIZE_MAX;
   int rc = CLMI_OK;
   if (hipSetDevice(e->device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
       hipMalloc(&e->d_rows[0], sizeof(RowsMember) * n) != hipSuccess ||
@@ -1518,10 +1561,11 @@ void clmi_ensemble_destroy(clmi_ensemble *e)
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  for (auto &g : e->groups) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
-  }
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (hipStream_t st : e->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
   for (auto *p : e->d_rows)
     if (p) (void)hipFree(p);
   if (e->d_upd) (void)hipFree(e->d_upd);
@@ -1551,25 +1595,11 @@ int clmi_ensemble_lf_evolve(clmi_ensemble *e, int nsteps)
   }
   for (clmi_ctx *c : e->m) EHIP(e, hipStreamSynchronize(c->stream));
   CHK(ens_tables(e));
-  for (auto &g : e->groups) {
-    // two steps (both buffer parities) per graph replay
-    if (nsteps >= 2 && !g.exec) {
-      EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
-      int rc = ens_step(e, g, 0);
-      if (rc == CLMI_OK) rc = ens_step(e, g, 1);
-      hipGraph_t gr = nullptr;
-      const hipError_t ce = hipStreamEndCapture(e->stream, &gr);
-      if (rc != CLMI_OK) {
-        if (gr) (void)hipGraphDestroy(gr);
-        return rc;
-      }
-      EHIP(e, ce);
-      g.graph = gr;
-      EHIP(e, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-    }
-    for (int k = 0; k + 1 < nsteps; k += 2) EHIP(e, hipGraphLaunch(g.exec, e->stream));
-    if (nsteps % 2) CHK(ens_step(e, g, 0));
-  }
+  // two steps (both buffer parities) of every group per graph replay
+  if (nsteps >= 2 && !e->exec) CHK(ens_capture(e));
+  for (int k = 0; k + 1 < nsteps; k += 2) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+  if (nsteps % 2)
+    for (const auto &g : e->groups) CHK(ens_step(e, g, 0, e->stream));
   EHIP(e, hipStreamSynchronize(e->stream));
   for (clmi_ctx *c : e->m) {
     if (nsteps % 2) {

@@ -221,11 +221,10 @@ int clmi_grf_u1(clmi_ctx *ctx, uint64_t seed, uint32_t stream, const double mass
 
 /* No counterpart in the reference (which runs one realisation per process).  Independent lattices --
  * same device, precision, number of scalar singlets and expansion setting; N = 64, 128 or 192 each;
- * potentials, dt, dx and initial conditions free -- advanced together.  Members of one size are
- * packed into groups whose evolved arrays total at most group_bytes (0: 96 MiB, about what the
- * last-level cache holds for reading and writing); every leapfrog step of a group is one batched
- * lattice launch plus one batched background launch, replayed from a captured HIP graph, and each
- * group runs all requested steps before the next starts.  Each member's fields and background are
+ * potentials, dt, dx and initial conditions free -- advanced together.  Members of one size form a
+ * group (split further when group_bytes > 0 caps a group's evolved arrays); every leapfrog step of a
+ * group is one batched lattice launch plus one batched background launch, the groups run as
+ * parallel branches of one captured HIP graph (two steps per replay).  Each member's fields and background are
  * bit-identical to clmi_lf_evolve on that member alone.  Members keep their own contexts for initial
  * conditions, observations and sync; the ensemble borrows them and must be destroyed before any
  * member. */

@@ -450,7 +450,7 @@ def test_grf_u1_parity(gpu, oracle_mod):
 
 
 @pytest.mark.parametrize("prec,group_bytes", [(CLMI_F32_BULK, 0), (CLMI_F64, 0), (CLMI_F32_BULK, 1),
-                                               (CLMI_F32_BULK, 1 << 40)])
+                                               (CLMI_F32_BULK, 64 << 20)])
 def test_ensemble_bit_identical_to_individual_runs(gpu, prec, group_bytes):
     """An ensemble of uneven lattices (64^3, 128^3, 192^3; different seeds, couplings and dt)
     advanced in batched, graph-replayed launches equals each lattice evolved on its own, bit for
