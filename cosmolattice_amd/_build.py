@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "scalar_rows_f64.hip",
     "scalar_rows_batch.hip",
     "gauge_pass.hip",
+    "gauge_tiled.hip",
     "background.hip",
     "fft_kernels.hip",
     "adjoint_kernels.hip",

@@ -48,6 +48,8 @@ struct clmi_ctx {
   TiledPlan tplan;
   bool rowk = false;   // scalar passes use the full-row kernel with plan rplan
   RowsPlan rplan;
+  bool gtiled = false; // charged scalar + U(1) passes use the tiled kernel (x-chunk gXC)
+  int gXC = 0;
   size_t partial_rows = 0;
 
   ScalarPoly spoly{};
@@ -258,7 +260,10 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     a.st = c->dstate;
     a.partials = c->partials + (size_t)r * kNumRed;
     a.poly = c->cpoly;
-    HIPCHK(c, launch_gauge_pass(c->f32, drift, kick, red, a, c->blocks_g, c->stream));
+    if (c->gtiled)
+      HIPCHK(c, launch_gauge_tiled(c->f32, drift, kick, red, a, c->blocks_g, c->gXC, c->stream));
+    else
+      HIPCHK(c, launch_gauge_pass(c->f32, drift, kick, red, a, c->blocks_g, c->stream));
     if (red) r += c->blocks_g;
     c->cur_fG = fo;
     c->cur_pG = po;
@@ -457,6 +462,12 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
           c->tplan = p;
           break;
         }
+    int gxc = 0, gblocks = 0;
+    if (g.n_cscalars > 0 && gauge_tiled_plan(g.N, kMaxPassBlocks, &gxc, &gblocks)) {
+      c->gtiled = true;
+      c->gXC = gxc;
+      c->blocks_g = gblocks;
+    }
   }
   size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
   if (c->tiled && (size_t)c->tplan.rows + c->blocks_g + 16 > rows) rows = (size_t)c->tplan.rows + c->blocks_g + 16;

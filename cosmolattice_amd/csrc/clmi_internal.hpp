@@ -164,6 +164,11 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
   }
 }
 
+// ---- tiled charged scalar + U(1) pass (gauge_tiled.hip) ----
+bool gauge_tiled_plan(int N, int max_blocks, int *XC, int *blocks);
+hipError_t launch_gauge_tiled(bool f32, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC,
+                              hipStream_t s);
+
 // ---- ensembles: several lattices in one launch (scalar_rows_batch.hip, background.hip) ----
 struct RowsMember {   // one lattice's fused pass inside a batched launch (device-resident table)
   ScalarPassArgs a;

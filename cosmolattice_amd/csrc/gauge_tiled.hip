@@ -1,0 +1,259 @@
+// Charged scalar + U(1) lattice pass, x-marching with LDS plane tiles (N a multiple of 64).
+//
+// A workgroup of four wavefronts owns a tile of 4 y-rows x 64 z-sites (one site per lane) and marches
+// along x over XC planes.  The drifted values of all five field components (phi_0, phi_1, A_x, A_y,
+// A_z) of planes x-1, x and x+1, each with a one-site y/z halo including corners, sit in a
+// three-slot LDS ring, so every neighbour the covariant Laplacian, the gauge Laplacian, the mixed
+// derivatives d-_j d+_i A_j (the diagonal sites x+i-j) and the links need is read from LDS; each
+// field value is loaded from memory and drifted once per tile instead of once per use.  The raw
+// values of plane x+2 are loaded while plane x is computed.  The per-site arithmetic is
+// gauge_site.hpp's, so fields are bit-identical to the one-site-per-thread pass; volume sums are
+// per workgroup (one partial row each).
+#include "gauge_site.hpp"
+
+namespace clmi {
+namespace {
+
+constexpr int GTZ = 64;  // z sites per tile (one per lane)
+constexpr int GTY = 4;   // y rows per tile (one per wavefront)
+constexpr int GNC = 5;   // components: phi_0, phi_1, A_x, A_y, A_z
+
+// Raw (not yet drifted) values of one site's five components.
+template <bool F32> struct GRaw;
+template <> struct GRaw<false> {
+  double f[GNC], p[GNC];
+  __device__ __forceinline__ void load(const GaugePassArgs &a, uint32_t s, bool u1, bool drift)
+  {
+    for (int q = 0; q < 2; ++q) {
+      f[q] = ((const double *)a.phi_in[q])[s];
+      p[q] = drift ? ((const double *)a.pi_in[q])[s] : 0.0;
+    }
+    for (int d = 0; d < 3; ++d) {
+      f[2 + d] = u1 ? ((const double *)a.A_in[d])[s] : 0.0;
+      p[2 + d] = (u1 && drift) ? ((const double *)a.piA_in[d])[s] : 0.0;
+    }
+  }
+  // drifted values exactly as GVal<false>::fld computes them
+  __device__ __forceinline__ double value(int q, bool drift, double cs, double cu) const
+  {
+    double v = f[q];
+    if (drift) v += (q < 2 ? cs : cu) * p[q];
+    return v;
+  }
+};
+template <> struct GRaw<true> {
+  float h[GNC], l[GNC], p[GNC];
+  __device__ __forceinline__ void load(const GaugePassArgs &a, uint32_t s, bool u1, bool drift)
+  {
+    for (int q = 0; q < 2; ++q) {
+      h[q] = ((const float *)a.phi_in[q])[s];
+      l[q] = a.philo_in[q][s];
+      p[q] = drift ? ((const float *)a.pi_in[q])[s] : 0.0f;
+    }
+    for (int d = 0; d < 3; ++d) {
+      h[2 + d] = u1 ? ((const float *)a.A_in[d])[s] : 0.0f;
+      l[2 + d] = u1 ? a.Alo_in[d][s] : 0.0f;
+      p[2 + d] = (u1 && drift) ? ((const float *)a.piA_in[d])[s] : 0.0f;
+    }
+  }
+  __device__ __forceinline__ P2 value(int q, bool drift, float cs, float cu) const
+  {
+    P2 v{h[q], l[q]};
+    if (drift) pair_add(v.h, v.l, (q < 2 ? cs : cu) * p[q]);
+    return v;
+  }
+};
+
+template <bool F32, bool DRIFT, bool KICK, bool RED>
+__global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, const int XC)
+{
+  using G = GVal<F32>;
+  using V = typename G::V;
+  using P = typename G::P;
+  using DT = typename std::conditional<F32, float, double>::type;  // drift factor type
+  __shared__ V lds[3][GNC][GTY + 2][GTZ + 2];
+
+  const int N = (int)a.N;
+  const uint32_t NN = (uint32_t)N * N;
+  const int nTz = N / GTZ, nTy = N / GTY;
+  const int nwg = gridDim.x, b = blockIdx.x;
+  const int L = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
+  const int tz = L % nTz, ty = (L / nTz) % nTy, cx = L / (nTz * nTy);
+  const int z0 = tz * GTZ, y0 = ty * GTY, x0 = cx * XC;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int y = y0 + wave, z = z0 + lane;
+  const GaugeK k = gauge_consts(a, KICK);
+  const bool u1 = k.u1;
+  DT cs = 0, cu = 0;
+  if (DRIFT) {
+    if constexpr (F32) {
+      cs = (float)(a.st->fdrift_s * a.cdt);
+      cu = (float)(a.st->fdrift_u * a.cdt);
+    } else {
+      cs = a.st->fdrift_s * a.cdt;
+      cu = a.st->fdrift_u * a.cdt;
+    }
+  }
+
+  auto wrap = [&](int v) -> int { return v < 0 ? v + N : (v >= N ? v - N : v); };
+  auto site = [&](int xx, int yy, int zz) -> uint32_t {
+    return (uint32_t)wrap(xx) * NN + (uint32_t)wrap(yy) * N + (uint32_t)wrap(zz);
+  };
+
+  // halo cell of this thread (140 of 256 threads): rows y0-1 and y0+4 with corners, columns z0-1, z0+64
+  int hly = -1, hlz = 0, hy = 0, hz = 0;
+  if (t < GTZ + 2) {
+    hly = 0; hlz = t; hy = y0 - 1; hz = z0 - 1 + t;
+  } else if (t < 2 * (GTZ + 2)) {
+    hly = GTY + 1; hlz = t - (GTZ + 2); hy = y0 + GTY; hz = z0 - 1 + hlz;
+  } else if (t < 2 * (GTZ + 2) + GTY) {
+    const int r = t - 2 * (GTZ + 2);
+    hly = r + 1; hlz = 0; hy = y0 + r; hz = z0 - 1;
+  } else if (t < 2 * (GTZ + 2) + 2 * GTY) {
+    const int r = t - 2 * (GTZ + 2) - GTY;
+    hly = r + 1; hlz = GTZ + 1; hy = y0 + r; hz = z0 + GTZ;
+  }
+  const bool halo = hly >= 0;
+
+  auto put = [&](int slot, const GRaw<F32> &own, const GRaw<F32> &hal, bool has_halo) {
+    for (int q = 0; q < GNC; ++q) {
+      lds[slot][q][wave + 1][lane + 1] = own.value(q, DRIFT, cs, cu);
+      if (has_halo) lds[slot][q][hly][hlz] = hal.value(q, DRIFT, cs, cu);
+    }
+  };
+  auto slot_of = [&](int xx) -> int { return (xx - x0 + 1) % 3; };
+
+  // prologue: planes x0-1, x0, x0+1
+  for (int xx = x0 - 1; xx <= x0 + 1; ++xx) {
+    GRaw<F32> own, hal;
+    own.load(a, site(xx, y, z), u1, DRIFT);
+    if (halo) hal.load(a, site(xx, hy, hz), u1, DRIFT);
+    put(slot_of(xx), own, hal, halo);
+  }
+  __syncthreads();
+
+  double acc[5] = {0, 0, 0, 0, 0};
+  const int ly = wave + 1, lz = lane + 1;
+  for (int i = 0; i < XC; ++i) {
+    const int x = x0 + i;
+    const bool more = i + 1 < XC;
+    GRaw<F32> pown, phal;
+    if (more) {
+      pown.load(a, site(x + 2, y, z), u1, DRIFT);
+      if (halo) phal.load(a, site(x + 2, hy, hz), u1, DRIFT);
+    }
+    const int sc = slot_of(x), sp = slot_of(x + 1), sm = slot_of(x - 1);
+    const auto &cur = lds[sc];
+    const auto &nx = lds[sp];
+    const auto &pv = lds[sm];
+    GaugeIn<V> g;
+    for (int q = 0; q < 2; ++q) {
+      g.fc[q] = cur[q][ly][lz];
+      g.fp[0][q] = nx[q][ly][lz];
+      g.fm[0][q] = pv[q][ly][lz];
+      g.fp[1][q] = cur[q][ly + 1][lz];
+      g.fm[1][q] = cur[q][ly - 1][lz];
+      g.fp[2][q] = cur[q][ly][lz + 1];
+      g.fm[2][q] = cur[q][ly][lz - 1];
+    }
+    for (int j = 0; j < 3; ++j) {
+      const int c = 2 + j;
+      g.Ac[j] = cur[c][ly][lz];
+      g.Ap[0][j] = nx[c][ly][lz];
+      g.Ap[1][j] = cur[c][ly + 1][lz];
+      g.Ap[2][j] = cur[c][ly][lz + 1];
+      g.Am[0][j] = pv[c][ly][lz];
+      g.Am[1][j] = cur[c][ly - 1][lz];
+      g.Am[2][j] = cur[c][ly][lz - 1];
+    }
+    // A_j(x + i - j), i != j
+    g.Ad[0][0] = g.Ad[1][1] = g.Ad[2][2] = V{};
+    g.Ad[0][1] = nx[3][ly - 1][lz];
+    g.Ad[0][2] = nx[4][ly][lz - 1];
+    g.Ad[1][0] = pv[2][ly + 1][lz];
+    g.Ad[1][2] = cur[4][ly + 1][lz - 1];
+    g.Ad[2][0] = pv[2][ly][lz + 1];
+    g.Ad[2][1] = cur[3][ly - 1][lz + 1];
+
+    const uint32_t s = site(x, y, z);
+    if (DRIFT) {
+      for (int q = 0; q < 2; ++q) G::store(a.phi_out[q], a.philo_out[q], s, g.fc[q]);
+      if (u1)
+        for (int d = 0; d < 3; ++d) G::store(a.A_out[d], a.Alo_out[d], s, g.Ac[d]);
+    }
+    P pin[2], pAin[3] = {0, 0, 0}, pout[2], pAout[3];
+    for (int q = 0; q < 2; ++q) pin[q] = ((const P *)a.pi_in[q])[s];
+    if (u1)
+      for (int d = 0; d < 3; ++d) pAin[d] = ((const P *)a.piA_in[d])[s];
+    if constexpr (F32)
+      gauge_site_f32<KICK, RED>(k, a.poly, g, pin, pAin, pout, pAout, acc);
+    else
+      gauge_site_f64<KICK, RED>(k, a.poly, g, pin, pAin, pout, pAout, acc);
+    if (KICK) {
+      for (int q = 0; q < 2; ++q) ((P *)a.pi_out[q])[s] = pout[q];
+      if (u1)
+        for (int d = 0; d < 3; ++d) ((P *)a.piA_out[d])[s] = pAout[d];
+    }
+    if (more) {
+      __syncthreads();  // every wavefront is done with slot sm (plane x - 1)
+      put(sm, pown, phal, halo);  // plane x + 2 takes its slot
+      __syncthreads();
+    }
+  }
+  if (RED) {
+    double v[kNumRed] = {0, 0, acc[2], acc[0], acc[1], acc[3], acc[4], 0};
+    block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
+  }
+}
+
+template <bool F32>
+hipError_t dispatch_tiled(bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC, hipStream_t s)
+{
+  const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
+#define CLMI_GT(D, K, R)                                                                              \
+  case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                    \
+    hipLaunchKernelGGL((k_gauge_tiled<F32, D, K, R>), dim3(blocks), dim3(256), 0, s, a, XC);          \
+    break;
+  switch (code) {
+    CLMI_GT(true, true, true)
+    CLMI_GT(true, true, false)
+    CLMI_GT(true, false, true)
+    CLMI_GT(true, false, false)
+    CLMI_GT(false, true, true)
+    CLMI_GT(false, true, false)
+    CLMI_GT(false, false, true)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef CLMI_GT
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Plan of the tiled pass: x-chunk XC (a divisor of N, at least 8) keeping the workgroup count within
+// max_blocks; false when N does not fit the tile.
+bool gauge_tiled_plan(int N, int max_blocks, int *XC, int *blocks)
+{
+  if (N % GTZ != 0 || N % GTY != 0) return false;
+  const long tiles = (long)(N / GTY) * (N / GTZ);
+  for (int xc = 8; xc <= N; xc *= 2) {
+    if (N % xc != 0) continue;
+    const long nb = tiles * (N / xc);
+    if (nb <= max_blocks) {
+      *XC = xc;
+      *blocks = (int)nb;
+      return true;
+    }
+  }
+  return false;
+}
+
+hipError_t launch_gauge_tiled(bool f32, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC,
+                              hipStream_t s)
+{
+  return f32 ? dispatch_tiled<true>(drift, kick, red, a, blocks, XC, s)
+             : dispatch_tiled<false>(drift, kick, red, a, blocks, XC, s);
+}
+
+}  // namespace clmi

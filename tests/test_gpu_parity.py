@@ -338,12 +338,14 @@ def load_u1(lat, st):
         lat.upload(k, 0, c, a)
 
 
-def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod):
+@pytest.mark.parametrize("N", [16, 64])
+def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod, N):
     """Charged scalar + U(1), expansion on, 40 leapfrog steps with measurements every 10: all ten
-    arrays and the background match the oracle within 1e-10; Gauss checks agree."""
-    say("case lf_lphi4U1_fp64")
+    arrays and the background match the oracle within 1e-10; Gauss checks agree (N = 16: the
+    one-site-per-thread pass; N = 64: the tiled pass)."""
+    say(f"case lf_lphi4U1_fp64 N={N}")
     spec = u1_spec()
-    N, dx, dt = 16, 0.6, 0.01
+    dx, dt = 0.6, 0.01
     rng = np.random.default_rng(31)
     st = u1_ics(spec, N, rng)
     lat = Lattice(spec, N, dx, dt, CLMI_F64)
@@ -360,6 +362,32 @@ def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod):
     np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
     gg, go = lat.gauss_u1(), orc.gauss_u1()
     np.testing.assert_allclose(gg, go, rtol=1e-9)
+
+
+@pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (64, CLMI_F32_BULK), (128, CLMI_F32_BULK), (128, CLMI_F64)])
+def test_tiled_gauge_pass_bit_identical_to_generic(gpu, N, prec):
+    """The tiled charged-scalar + U(1) pass gives the same fields, bit for bit, as the
+    one-site-per-thread pass (flag CLMI_FLAG_NO_TILED); volume averages agree to summation order."""
+    say(f"case gauge tiled N={N} prec={prec}")
+    spec = u1_spec()
+    dx, dt = 0.6, 0.01
+    rng = np.random.default_rng(35)
+    st = u1_ics(spec, N, rng)
+    res = []
+    for flags in (0, 1):
+        lat = Lattice(spec, N, dx, dt, prec, expansion=False, flags=flags)
+        load_u1(lat, st)
+        lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=0.0)
+        lat.lf_init()
+        lat.lf_evolve(7)
+        lat.lf_sync()
+        res.append(({kc: lat.download(kc[0], 0, kc[1]) for kc in st}, lat.averages(), lat.gauss_u1()))
+        lat.close()
+    for kc in st:
+        assert np.array_equal(res[0][0][kc], res[1][0][kc]), kc
+    for key in res[0][1]:
+        assert res[0][1][key] == pytest.approx(res[1][1][key], rel=1e-12, abs=1e-300), key
+    np.testing.assert_allclose(res[0][2], res[1][2], rtol=1e-9)
 
 
 def test_lphi4u1_gauss_conserved_fp64_and_f32(gpu):
