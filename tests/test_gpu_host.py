@@ -5,6 +5,7 @@ Each case writes the initial state as raw fields, runs the model program with a 
 (measurement cadence, evolver), reads back its dump and its average_energy_conservation.txt, and
 replays the same loop on the oracle: sync + Friedmann check at every output step, then evolve.
 """
+import json
 import math
 import subprocess
 from pathlib import Path
@@ -36,17 +37,26 @@ def lphi4_params(N, dx, dt, steps, out_every, evolver="LF", extra="", geometry=N
             f"initial_momenta = {-0.5 * phi0 * phi0!r} 0\n{extra}")
 
 
-def write_state(d: Path, arrays: dict, bg: dict):
+KIND_NAMES = {0: "fldS", 1: "piS", 2: "fldCS", 3: "piCS", 6: "fldU1", 7: "piU1"}
+
+
+def write_state(d: Path, arrays: dict, bg: dict, N: int, t: float = 0.0):
+    """The saved-state directory the host programs read (raw components + state.json)."""
     d.mkdir(parents=True, exist_ok=True)
     for (k, i, c), a in arrays.items():
-        np.ascontiguousarray(a, dtype="<f8").tofile(d / f"{k}_{i}_{c}.f64")
-    (d / "background.txt").write_text("\n".join(repr(float(bg.get(n, 0.0))) for n in BG_NAMES) + "\n")
+        np.ascontiguousarray(a, dtype="<f8").tofile(d / f"{KIND_NAMES[k]}_{i}_{c}.f64")
+    state = {"model": "test", "N": N, "tSave": t}
+    for n in ("aSI", "aI", "aIM", "aDotSI", "aDotI", "aDotSIM"):
+        state[n] = float(bg.get(n, 0.0))
+    state["background"] = {f"bg_{n}": float(bg.get(n, 0.0)) for n in BG_NAMES}
+    (d / "state.json").write_text(json.dumps(state))
 
 
 def read_state(d: Path, keys, N):
-    arrays = {key: np.fromfile(d / f"{key[0]}_{key[1]}_{key[2]}.f64", dtype="<f8").reshape(N, N, N) for key in keys}
-    vals = [float(x) for x in (d / "background.txt").read_text().split()]
-    return arrays, dict(zip(BG_NAMES, vals))
+    arrays = {key: np.fromfile(d / f"{KIND_NAMES[key[0]]}_{key[1]}_{key[2]}.f64", dtype="<f8").reshape(N, N, N)
+              for key in keys}
+    state = json.loads((d / "state.json").read_text())
+    return arrays, {n: state["background"][f"bg_{n}"] for n in BG_NAMES}
 
 
 def run_program(model, tmp_path, params, *args):
@@ -114,7 +124,7 @@ def test_host_program_matches_oracle(gpu, oracle_mod, tmp_path, evolver):
     orc.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=1.0)
     _, _, rhs = orc.hubble_constraint()
     orc.set_background(aDotI=math.sqrt(rhs))
-    write_state(tmp_path / "in", {(k, i, 0): a for (k, i), a in ics.items()}, orc.background())
+    write_state(tmp_path / "in", {(k, i, 0): a for (k, i), a in ics.items()}, orc.background(), N)
     run_program("lphi4", tmp_path, lphi4_params(N, dx, dt, steps, every, evolver),
                 f"clmi_load_dir={tmp_path / 'in'}", f"clmi_dump_dir={tmp_path}")
     obs = []
@@ -217,3 +227,31 @@ def test_host_lphi4u1_initializer(gpu, oracle_mod, tmp_path):
     for (k, i, c), a in got.items():
         orc.upload(k, i, c, a)
     assert orc.gauss_u1()[0] < 1e-13
+
+
+def test_host_backup_and_restart(gpu, tmp_path):
+    """A run restarted from its own backup (load_file, as SimulationManager::loadSim) ends where the
+    uninterrupted run ends, to rounding: the restart recomputes the cached averages
+    (setAllAverages) and skips the LeapFrog start-up half step, as the reference does."""
+    say("case host restart")
+    N, dx, dt = 16, 0.4909, 0.005
+    spec = bench_spec()
+    params = lphi4_params(N, dx, dt, 60, 60, extra="kCutOff = 3.0\nbaseSeed = 5\n") + \
+        f"tBackupFreq = {20 * dt!r}\n"
+    full = tmp_path / "full"
+    full.mkdir()
+    run_program("lphi4", full, params, f"backup_dir={full}/", f"clmi_dump_dir={full}/end")
+    backup = full / "lphi4_backup"
+    assert (backup / "state.json").exists() and (full / "lphi4_backup~").exists()
+    st = json.loads((backup / "state.json").read_text())
+    assert st["tSave"] == pytest.approx(41 * dt)
+    rest = tmp_path / "rest"
+    rest.mkdir()
+    run_program("lphi4", rest, params, f"load_file={backup}", f"clmi_dump_dir={rest}/end")
+    keys = [(FLD_S, 0, 0), (PI_S, 0, 0), (FLD_S, 1, 0), (PI_S, 1, 0)]
+    a, bga = read_state(full / "end", keys, N)
+    b, bgb = read_state(rest / "end", keys, N)
+    for k in keys:
+        assert np.max(np.abs(a[k] - b[k])) <= 1e-11 * np.max(np.abs(a[k])), k
+    assert bga["aI"] == pytest.approx(bgb["aI"], rel=1e-12)
+    assert json.loads((rest / "end" / "state.json").read_text())["tSave"] == pytest.approx(60 * dt)
