@@ -100,6 +100,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_lf_sync": (C.c_int, [P]),
         "clmi_lf_is_synced": (C.c_int, [P]),
         "clmi_vv_evolve": (C.c_int, [P, C.POINTER(C.c_double), C.c_int, C.c_int]),
+        "clmi_pv_evolve": (C.c_int, [P, C.POINTER(C.c_double), C.c_int, C.c_int]),
         "clmi_kick": (C.c_int, [P, C.c_double]),
         "clmi_drift": (C.c_int, [P, C.c_double]),
         "clmi_set_all_averages": (C.c_int, [P]),
@@ -354,6 +355,10 @@ class Lattice:
         w = np.ascontiguousarray(ws, dtype=np.float64)
         self._check(self.lib.clmi_vv_evolve(self._h, _dptr(w), len(w), nsteps), "vv_evolve")
 
+    def pv_evolve(self, ws: Sequence[float], nsteps: int = 1):
+        w = np.ascontiguousarray(ws, dtype=np.float64)
+        self._check(self.lib.clmi_pv_evolve(self._h, _dptr(w), len(w), nsteps), "pv_evolve")
+
     def kick(self, c: float):
         self._check(self.lib.clmi_kick(self._h, c), "kick")
 
@@ -544,11 +549,25 @@ class LeapFrog:
         return self.lat.is_synced()
 
 
-# Composition weights of velocityverletparameters.h:26-27 (VV2) and :28-29 (VV4).
-VV_WEIGHTS = {
-    "VV2": [1.0],
-    "VV4": [1.351207191959657771818, -1.702414403875838200264],
+# Symmetric composition weights w[0..n-1] of the Verlet schemes; a step runs the stages w[0], ...,
+# w[n-1], ..., w[0].  Velocity Verlet (evolvers/velocityverletparameters.h:23-47) and position Verlet
+# (evolvers/positionverletparameters.h:22-42) use the same published coefficients: Yoshida (1990) for
+# orders 4 and 6, Kahan and Li (1997) for 6 (alternative), 8 and 10.
+_COMPOSITION = {
+    "2": [1.0],
+    "4": [1.351207191959657771818, -1.702414403875838200264],
+    "6": [0.78451361047755726382, 0.23557321335935813368, -1.1776799841788710069, 1.3151863206839112189],
+    "6_2": [0.39216144400731413928, 0.33259913678935943860, -0.70624617255763935981, 0.082213596293550800230,
+            0.79854399093482996340],
+    "8": [0.74167036435061295345, -0.40910082580003159400, 0.19075471029623837995, -0.57386247111608226666,
+          0.29906418130365592384, 0.33462491824529818378, 0.31529309239676659663, -0.79688793935291635402],
+    "10": [-0.48159895600253002870, 0.0036303931544595926879, 0.50180317558723140279, 0.28298402624506254868,
+           0.80702967895372223806, -0.026090580538592205447, -0.87286590146318071547, -0.52373568062510581643,
+           0.44521844299952789252, 0.18612289547097907887, 0.23137327866438360633, -0.52191036590418628905,
+           0.74866113714499296793, 0.066736511890604057532, -0.80360324375670830316, 0.91249037635867994571],
 }
+VV_WEIGHTS = {"VV" + k: v for k, v in _COMPOSITION.items()}
+PV_WEIGHTS = {"PV" + k: v for k, v in _COMPOSITION.items()}
 
 
 class VelocityVerlet:
@@ -563,3 +582,17 @@ class VelocityVerlet:
 
     def sync(self):
         pass  # fields and momenta live at integer times (velocityverlet.h:119-122)
+
+
+class PositionVerlet:
+    """PositionVerlet evolver (include/CosmoInterface/evolvers/positionverlet.h:29-296)."""
+
+    def __init__(self, lat: Lattice, kind: str = "PV2"):
+        self.lat = lat
+        self.ws = PV_WEIGHTS[kind]
+
+    def evolve(self, nsteps: int = 1):
+        self.lat.pv_evolve(self.ws, nsteps)
+
+    def sync(self):
+        pass  # only a fixed background needs it (positionverlet.h:115-119); not supported here

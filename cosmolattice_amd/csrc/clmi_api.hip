@@ -686,6 +686,30 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
   return CLMI_OK;
 }
 
+int clmi_pv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
+{
+  CHK(check_ctx(c));
+  if (!ws || nws < 1 || nsteps < 0) return fail(c, CLMI_E_INVALID, "bad composition weights");
+  CHK(flush(c));
+  const bool ex = c->cfg.expansion != 0;
+  const double dt = c->cfg.dt;
+  const int stages = (nws - 1) * 2 + 1;
+  if (!ex) CHK(update(c, 0, {{OP_PV_DRIFT_AI, 0.0}}));
+  for (int n = 0; n < nsteps; ++n)
+    for (int s = 0; s < stages; ++s) {
+      const double w = ws[s < nws ? s : stages - s - 1];
+      // drift (aI of the stage start) + kick (half-step aI) in one pass, then the mid-stage update
+      if (ex) CHK(update(c, 0, {{OP_PV_SF_FIRST, w}}));
+      CHK(pass_update(c, true, true, ex, dt / 2 * w, w * dt, w * dt, ex, {{OP_PV_MID, w}}));
+      // second drift (new aI) + storeFieldsAveragesInt
+      CHK(pass_update(c, true, false, ex, dt / 2 * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
+    }
+  // static background: restore the aSI-based drift coefficients of the other evolvers (with
+  // expansion the last update already did)
+  if (!ex) CHK(update(c, 0, {}));
+  return CLMI_OK;
+}
+
 int clmi_kick(clmi_ctx *c, double ck)
 {
   CHK(check_ctx(c));

@@ -82,6 +82,10 @@ enum UpdateOp : int {
   OP_VV_SF_ONE = 8,      // kickScaleFactorOne         (velocityverlet.h:171-174)
   OP_SET_ALL = 9,        // Averages::setAllAverages   (averages.h:86-129)
   OP_KERNEL_ONLY = 10,   // recompute K only
+  OP_PV_SF_FIRST = 11,   // driftScaleFactorFirstHalf  (positionverlet.h:196-209)
+  OP_PV_MID = 12,        // storeFieldsAveragesHalf + storeMomentaAverages + kickScaleFactor +
+                         // driftScaleFactorSecondHalf (positionverlet.h:73-94)
+  OP_PV_DRIFT_AI = 13,   // drift coefficients from aI (positionverlet.h:161-192)
 };
 
 struct UpdateOps {

@@ -27,29 +27,35 @@ __device__ inline double rpow(double a, double e)
   return pow(a, e);
 }
 
-// ScaleFactorKernels::get (evolvers/kernels/scalefactorkernels.h:27-81) with the energy
-// normalisations of definitions/energies.h:37-85.
-__device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &mc)
+// The scale-factor kernel of ScaleFactorKernels::get/getSI (evolvers/kernels/scalefactorkernels.h:27-124)
+// with the energy normalisations of definitions/energies.h:37-85, from the given volume averages.
+__device__ inline double sf_kernel_of(double a, double pi2S, double grad2S, double pi2CS, double grad2CS,
+                                      double pi2U1, double mag2U1, double pot, const ModelConsts &mc)
 {
-  const double a = b.aI;
   double Eks = 0, Egs = 0, Ekcs = 0, Egcs = 0, Eel = 0, Emag = 0;
   if (mc.ns > 0) {
-    Eks = 0.5 * b.pi2AvI * rpow(a, -6.0);
-    Egs = 0.5 * b.grad2AvI * rpow(a, -2.0);
+    Eks = 0.5 * pi2S * rpow(a, -6.0);
+    Egs = 0.5 * grad2S * rpow(a, -2.0);
   }
   if (mc.ncs > 0) {
-    Ekcs = b.CSpi2AvI * rpow(a, -6.0);
-    Egcs = b.CSgrad2AvI * rpow(a, -2.0);
+    Ekcs = pi2CS * rpow(a, -6.0);
+    Egcs = grad2CS * rpow(a, -2.0);
   }
   if (mc.nu1 > 0) {
     const double w2 = (mc.omegaStar / mc.fStar) * (mc.omegaStar / mc.fStar);
-    Eel = 0.5 * w2 * rpow(a, -4.0) * b.U1pi2AvI;
-    Emag = 0.5 * w2 * rpow(a, -4.0) * b.U1Mag2AvI;
+    Eel = 0.5 * w2 * rpow(a, -4.0) * pi2U1;
+    Emag = 0.5 * w2 * rpow(a, -4.0) * mag2U1;
   }
   const double fm = mc.fStar / mc.MPl;
   return rpow(a, 2 * mc.alpha + 1) / 3.0 * (fm * fm) *
          ((mc.alpha - 2) * (Eks + Ekcs) + mc.alpha * (Egs + Egcs) + (mc.alpha - 1) * (Eel + Emag) +
-          (mc.alpha + 1) * b.potAvI);
+          (mc.alpha + 1) * pot);
+}
+
+// ScaleFactorKernels::get: integer-time averages.
+__device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &mc)
+{
+  return sf_kernel_of(b.aI, b.pi2AvI, b.grad2AvI, b.CSpi2AvI, b.CSgrad2AvI, b.U1pi2AvI, b.U1Mag2AvI, b.potAvI, mc);
 }
 
 // Fixed-order sum of `rows` partial rows by a block of NT threads: thread t takes rows t, t + NT,
@@ -98,6 +104,7 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
 {
   clmi_background &b = st->bg;
   const double dt = mc.dt;
+  double a_drift = -1.0;  // > 0: the drift coefficients use this scale factor instead of aSI
   for (int o = 0; o < ops.n; ++o) {
     const double w = ops.w[o];
     switch (ops.op[o]) {
@@ -182,6 +189,36 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
       case OP_KERNEL_ONLY:
         st->K = sf_kernel(b, mc);
         break;
+      case OP_PV_SF_FIRST:  // positionverlet.h:196-209; the first drift of the stage uses aI before it
+        a_drift = b.aI;
+        b.aIM = b.aI;
+        b.aI += dt / 2.0 * b.aDotI * w;
+        b.aSI = b.aI;
+        break;
+      case OP_PV_MID: {  // positionverlet.h:73-94
+        // storeFieldsAveragesHalf (positionverlet.h:258-266): the half-step field averages feed getSI
+        const double g2SI = red[RED_GRAD2S], cg2SI = red[RED_GRAD2CS], b2SI = red[RED_B2U1];
+        b.potAvSI = red[RED_POT];
+        // storeMomentaAverages (positionverlet.h:238-256)
+        const double pIM = b.pi2AvI, cIM = b.CSpi2AvI, uIM = b.U1pi2AvI;
+        if (mc.ns > 0) b.pi2AvI = red[RED_PI2S];
+        if (mc.ncs > 0) b.CSpi2AvI = red[RED_PI2CS];
+        if (mc.nu1 > 0) b.U1pi2AvI = red[RED_PI2U1];
+        b.pi2AvSI = 0.5 * (b.pi2AvI + pIM);
+        b.CSpi2AvSI = 0.5 * (b.CSpi2AvI + cIM);
+        b.U1pi2AvSI = 0.5 * (b.U1pi2AvI + uIM);
+        // kickScaleFactor with ScaleFactorKernels::getSI (positionverlet.h:151-154)
+        st->K = sf_kernel_of(b.aI, b.pi2AvSI, g2SI, b.CSpi2AvSI, cg2SI, b.U1pi2AvSI, b2SI, b.potAvSI, mc);
+        b.aDotI += dt * st->K * w;
+        // driftScaleFactorSecondHalf (positionverlet.h:212-223); the second drift uses the new aI
+        b.aI += dt / 2.0 * b.aDotI * w;
+        b.aSI = (b.aIM + b.aI) / 2.0;
+        a_drift = b.aI;
+        break;
+      }
+      case OP_PV_DRIFT_AI:
+        a_drift = b.aI;
+        break;
       default:
         break;
     }
@@ -190,8 +227,9 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
   st->aV = rpow(b.aI, 3 + mc.alpha);
   st->aG = rpow(b.aI, mc.alpha - 1);
   st->aJ = rpow(b.aI, 1 + mc.alpha);
-  st->fdrift_s = rpow(b.aSI, mc.alpha - 3);
-  st->fdrift_u = rpow(b.aSI, mc.alpha - 1);
+  const double ad = a_drift > 0.0 ? a_drift : b.aSI;
+  st->fdrift_s = rpow(ad, mc.alpha - 3);
+  st->fdrift_u = rpow(ad, mc.alpha - 1);
 }
 
 }  // namespace clmi

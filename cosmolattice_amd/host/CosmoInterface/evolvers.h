@@ -1,5 +1,5 @@
 // Evolvers with the reference's public API (evolvers/leapfrog.h:23-288, evolvers/velocityverlet.h:33-278,
-// evolvers/evolver.h:40-160).  Each call is one C-ABI call into libclmi; the lattice work runs on the
+// evolvers/positionverlet.h:29-296, evolvers/evolver.h:40-160).  Each call is one C-ABI call into libclmi; the lattice work runs on the
 // GPU, asynchronously, on the model's context stream.
 #pragma once
 
@@ -17,6 +17,11 @@ public:
   using std::runtime_error::runtime_error;
 };
 class NotVVEvolver : public std::runtime_error
+{
+public:
+  using std::runtime_error::runtime_error;
+};
+class NotPVEvolver : public std::runtime_error
 {
 public:
   using std::runtime_error::runtime_error;
@@ -66,6 +71,28 @@ template <typename T> struct VelocityVerletParameters {
   static bool isVerlet(EvolverType ev)
   {
     return ev == VV2 || ev == VV4 || ev == VV6 || ev == VV8 || ev == VV10 || ev == VV6_2;
+  }
+};
+
+// Position Verlet uses the same composition weights (evolvers/positionverletparameters.h:18-73).
+template <typename T> struct PositionVerletParameters {
+  static EvolverType asVV(EvolverType ev)
+  {
+    switch (ev) {
+      case PV2: return VV2;
+      case PV4: return VV4;
+      case PV6: return VV6;
+      case PV6_2: return VV6_2;
+      case PV8: return VV8;
+      case PV10: return VV10;
+      default: throw NotPVEvolver("not a position-Verlet evolver");
+    }
+  }
+  static std::vector<T> getCoeffs(EvolverType ev) { return VelocityVerletParameters<T>::getCoeffs(asVV(ev)); }
+  static size_t getOrder(EvolverType ev) { return VelocityVerletParameters<T>::getOrder(asVV(ev)); }
+  static bool isVerlet(EvolverType ev)
+  {
+    return ev == PV2 || ev == PV4 || ev == PV6 || ev == PV8 || ev == PV10 || ev == PV6_2;
   }
 };
 
@@ -128,6 +155,33 @@ public:
   bool expansion;
 };
 
+template <typename T> class PositionVerlet
+{
+public:
+  template <class Model>
+  PositionVerlet(Model &model, RunParameters<T> &rPar)
+      : ws(PositionVerletParameters<T>::getCoeffs(rPar.eType)), expansion(rPar.expansion)
+  {
+    model.attachDevice(rPar);
+    model.pushBackground();
+  }
+  // positionverlet.h:47-111, one step of dt.
+  template <class Model> void evolve(Model &model, T /*tMinust0*/, bool /*evolveGWs*/ = false)
+  {
+    evolveSteps(model, 1);
+  }
+  template <class Model> void evolveSteps(Model &model, int nsteps)
+  {
+    std::vector<double> w(ws.begin(), ws.end());
+    model.check(clmi_pv_evolve(model.context(), w.data(), (int)w.size(), nsteps), "clmi_pv_evolve");
+  }
+  // positionverlet.h:115-119: only a fixed background changes aDotI here; refresh the host copy.
+  template <class Model> void sync(Model &model, T /*tMinust0*/) { model.pullBackground(); }
+
+  std::vector<T> ws;
+  bool expansion;
+};
+
 // Dispatcher of evolvers/evolver.h:40-160 for the evolvers this library implements.
 template <class Model> class Evolver
 {
@@ -140,30 +194,38 @@ public:
       lf = std::make_unique<LeapFrog<T>>(model, rPar);
     else if (VelocityVerletParameters<T>::isVerlet(type))
       vv = std::make_unique<VelocityVerlet<T>>(model, rPar);
+    else if (PositionVerletParameters<T>::isVerlet(type))
+      pv = std::make_unique<PositionVerlet<T>>(model, rPar);
     else
       throw EvolverTypeNotInEvolver(
-          "The evolver type you specified is not implemented by this library (LF and VV2-VV10 are).");
+          "The evolver type you specified is not implemented by this library (LF, VV2-VV10 and PV2-PV10 are).");
   }
   void evolve(Model &model, T tMinust0) const
   {
     if (lf)
       lf->evolve(model, tMinust0);
-    else
+    else if (vv)
       vv->evolve(model, tMinust0);
+    else
+      pv->evolve(model, tMinust0);
   }
   void sync(Model &model, T tMinust0) const
   {
     if (lf)
       lf->sync(model, tMinust0);
-    else
+    else if (vv)
       vv->sync(model, tMinust0);
+    else
+      pv->sync(model, tMinust0);
   }
   void evolveSteps(Model &model, int nsteps) const
   {
     if (lf)
       lf->evolveSteps(model, nsteps);
-    else
+    else if (vv)
       vv->evolveSteps(model, nsteps);
+    else
+      pv->evolveSteps(model, nsteps);
   }
 
   EvolverType type;
@@ -171,6 +233,7 @@ public:
 private:
   std::unique_ptr<LeapFrog<T>> lf;
   std::unique_ptr<VelocityVerlet<T>> vv;
+  std::unique_ptr<PositionVerlet<T>> pv;
 };
 
 }  // namespace TempLat

@@ -147,6 +147,10 @@ int clmi_lf_is_synced(const clmi_ctx *ctx);
 /* nsteps x VelocityVerlet::evolve with composition weights ws[0..nws-1]
  * (evolvers/velocityverlet.h:51-114; VV2 = {1.0}, velocityverletparameters.h:26-27). */
 int clmi_vv_evolve(clmi_ctx *ctx, const double *ws, int nws, int nsteps);
+/* nsteps x PositionVerlet::evolve with composition weights ws[0..nws-1]
+ * (evolvers/positionverlet.h:47-111; PV2 = {1.0}, positionverletparameters.h:22-23).  Each stage is
+ * one drift+kick pass and one drift pass. */
+int clmi_pv_evolve(clmi_ctx *ctx, const double *ws, int nws, int nsteps);
 
 /* ---- primitives (one lattice pass each) ----------------------------------------------------- */
 

@@ -516,13 +516,14 @@ void orc_kick(orc_model *m, double c)
   if (m->nu1 > 0) kick_u1(m, c);
 }
 
-/* fldS += aSI^(alpha-3) * (cdt * piS); fldCS likewise; fldU1 += aSI^(alpha-1) * cdt * piU1
- * (evolvers/leapfrog.h:185-217, velocityverlet.h:181-209). */
-void orc_drift(orc_model *m, double cdt)
+/* fldS += a^(alpha-3) * (cdt * piS); fldCS likewise; fldU1 += a^(alpha-1) * cdt * piU1.  The leapfrog
+ * and velocity Verlet drift with a = aSI (evolvers/leapfrog.h:185-217, velocityverlet.h:181-209), the
+ * position Verlet with a = aI (evolvers/positionverlet.h:161-192). */
+static void drift_at(orc_model *m, double cdt, double a)
 {
   const size_t V = (size_t)m->N * m->N * m->N;
-  const double fs = pow(m->bg.aSI, m->alpha - 3);
-  const double fu = pow(m->bg.aSI, m->alpha - 1);
+  const double fs = pow(a, m->alpha - 3);
+  const double fu = pow(a, m->alpha - 1);
   for (int n = 0; n < m->ns; ++n) {
     double *phi = m->fldS[n];
     const double *pi = m->piS[n];
@@ -540,6 +541,8 @@ void orc_drift(orc_model *m, double cdt)
       for (size_t i = 0; i < V; ++i) m->fldU1[d][i] += fu * cdt * m->piU1[d][i];
     }
 }
+
+void orc_drift(orc_model *m, double cdt) { drift_at(m, cdt, m->bg.aSI); }
 
 /* ------------------------------------------------------------------------------------------ */
 /* Leapfrog (evolvers/leapfrog.h)                                                              */
@@ -652,6 +655,82 @@ void orc_vv_evolve(orc_model *m, const double *ws, int nws)
       store_momenta_averages_vv(m);
       b->aDotI = b->aDotSI + (m->dt / 2) * orc_scale_factor_kernel(m) * w;
     }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Position Verlet (evolvers/positionverlet.h:47-111)                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* ScaleFactorKernels::getSI (evolvers/kernels/scalefactorkernels.h:84-124): the kernel of get() with
+ * the half-step averages, the current aI in every energy normalisation (definitions/energies.h:37-85,
+ * 104-145). */
+static double scale_factor_kernel_si(const orc_model *m, double grad2SI, double CSgrad2SI, double U1Mag2SI)
+{
+  const orc_background *b = &m->bg;
+  const double a = b->aI;
+  double Eks = 0, Egs = 0, Ekcs = 0, Egcs = 0, Eel = 0, Emag = 0;
+  if (m->ns > 0) {
+    Eks = 0.5 * b->pi2AvSI * pow(a, -6);
+    Egs = 0.5 * grad2SI * pow(a, -2);
+  }
+  if (m->ncs > 0) {
+    Ekcs = b->CSpi2AvSI * pow(a, -6);
+    Egcs = CSgrad2SI * pow(a, -2);
+  }
+  if (m->nu1 > 0) {
+    double w2 = (m->omegaStar / m->fStar) * (m->omegaStar / m->fStar);
+    Eel = 0.5 * w2 * pow(a, -4) * b->U1pi2AvSI;
+    Emag = 0.5 * w2 * pow(a, -4) * U1Mag2SI;
+  }
+  double fm = m->fStar / m->MPl;
+  return pow(a, 2 * m->alpha + 1) / 3.0 * (fm * fm) *
+         ((m->alpha - 2) * (Eks + Ekcs) + m->alpha * (Egs + Egcs) + (m->alpha - 1) * (Eel + Emag) +
+          (m->alpha + 1) * b->potAvSI);
+}
+
+/* One step of PositionVerlet::evolve: for each stage weight w, drift by w*dt/2 (with aI), the
+ * half-step scale-factor drift, half-step field averages, kick by w*dt (with the half-step aI),
+ * storeMomentaAverages (positionverlet.h:238-256: previous integer-time value and the new one give
+ * the half-step value), kickScaleFactor with getSI, second half-step scale-factor drift
+ * (aSI = (aIM + aI)/2), drift by w*dt/2 with the new aI, integer-time field averages. */
+void orc_pv_evolve(orc_model *m, const double *ws, int nws)
+{
+  orc_background *b = &m->bg;
+  const int stages = (nws - 1) * 2 + 1;
+  const double dt = m->dt;
+  for (int s = 0; s < stages; ++s) {
+    const double w = ws[s < nws ? s : stages - s - 1];
+    drift_at(m, dt / 2.0 * w, b->aI);
+    double grad2SI = 0, CSgrad2SI = 0, U1Mag2SI = 0;
+    if (m->expansion) {
+      b->aIM = b->aI; /* driftScaleFactorFirstHalf (positionverlet.h:196-209) */
+      b->aI += dt / 2.0 * b->aDotI * w;
+      b->aSI = b->aI;
+      /* storeFieldsAveragesHalf (positionverlet.h:258-266) */
+      if (m->ns > 0) grad2SI = orc_avg_grad2S(m);
+      if (m->ncs > 0) CSgrad2SI = orc_avg_grad2CS(m);
+      if (m->nu1 > 0) U1Mag2SI = orc_avg_B2U1(m);
+      b->potAvSI = orc_avg_potential(m);
+    }
+    orc_kick(m, w * dt);
+    if (m->expansion) {
+      /* storeMomentaAverages (positionverlet.h:238-256) */
+      const double pIM = b->pi2AvI, cIM = b->CSpi2AvI, uIM = b->U1pi2AvI;
+      if (m->ns > 0) b->pi2AvI = orc_avg_pi2S(m);
+      if (m->ncs > 0) b->CSpi2AvI = orc_avg_pi2CS(m);
+      if (m->nu1 > 0) b->U1pi2AvI = orc_avg_pi2U1(m);
+      b->pi2AvSI = 0.5 * (b->pi2AvI + pIM);
+      b->CSpi2AvSI = 0.5 * (b->CSpi2AvI + cIM);
+      b->U1pi2AvSI = 0.5 * (b->U1pi2AvI + uIM);
+      /* kickScaleFactor (positionverlet.h:151-154) */
+      b->aDotI += dt * scale_factor_kernel_si(m, grad2SI, CSgrad2SI, U1Mag2SI) * w;
+      /* driftScaleFactorSecondHalf (positionverlet.h:212-223) */
+      b->aI += dt / 2.0 * b->aDotI * w;
+      b->aSI = (b->aIM + b->aI) / 2.0;
+    }
+    drift_at(m, dt / 2.0 * w, b->aI);
+    if (m->expansion) store_fields_averages(m); /* storeFieldsAveragesInt (positionverlet.h:268-276) */
   }
 }
 

@@ -33,6 +33,24 @@ MAX_SCALARS = 4
 MAX_TERMS = 8
 
 
+# Symmetric composition weights of the position-Verlet schemes (evolvers/positionverletparameters.h:22-42;
+# Yoshida 1990 for orders 4 and 6, Kahan and Li 1997 for 6_2, 8 and 10).  A step runs w[0..n-1] then
+# w[n-2..0].
+PV_WEIGHTS = {
+    "PV2": [1.0],
+    "PV4": [1.351207191959657771818, -1.702414403875838200264],
+    "PV6": [0.78451361047755726382, 0.23557321335935813368, -1.1776799841788710069, 1.3151863206839112189],
+    "PV6_2": [0.39216144400731413928, 0.33259913678935943860, -0.70624617255763935981, 0.082213596293550800230,
+              0.79854399093482996340],
+    "PV8": [0.74167036435061295345, -0.40910082580003159400, 0.19075471029623837995, -0.57386247111608226666,
+            0.29906418130365592384, 0.33462491824529818378, 0.31529309239676659663, -0.79688793935291635402],
+    "PV10": [-0.48159895600253002870, 0.0036303931544595926879, 0.50180317558723140279, 0.28298402624506254868,
+             0.80702967895372223806, -0.026090580538592205447, -0.87286590146318071547, -0.52373568062510581643,
+             0.44521844299952789252, 0.18612289547097907887, 0.23137327866438360633, -0.52191036590418628905,
+             0.74866113714499296793, 0.066736511890604057532, -0.80360324375670830316, 0.91249037635867994571],
+}
+
+
 class OrcBackground(C.Structure):
     _fields_ = [(n, C.c_double) for n in [
         "aI", "aSI", "aIM", "aDotI", "aDotSI", "aDotSIM",
@@ -78,7 +96,7 @@ def load() -> C.CDLL:
     M = C.POINTER(OrcModel)
     for name, res, args in [
         ("orc_lf_init", None, [M]), ("orc_lf_evolve", None, [M]), ("orc_lf_sync", None, [M]),
-        ("orc_vv_evolve", None, [M, DP, C.c_int]),
+        ("orc_vv_evolve", None, [M, DP, C.c_int]), ("orc_pv_evolve", None, [M, DP, C.c_int]),
         ("orc_kick", None, [M, C.c_double]), ("orc_drift", None, [M, C.c_double]),
         ("orc_set_all_averages", None, [M]),
         ("orc_avg_pi2S", C.c_double, [M]), ("orc_avg_grad2S", C.c_double, [M]),
@@ -184,6 +202,11 @@ class OracleLattice:
         w = np.ascontiguousarray(ws, dtype=np.float64)
         for _ in range(nsteps):
             self.lib.orc_vv_evolve(C.byref(self.m), _p(w), len(w))
+
+    def pv_evolve(self, ws, nsteps: int = 1):
+        w = np.ascontiguousarray(ws, dtype=np.float64)
+        for _ in range(nsteps):
+            self.lib.orc_pv_evolve(C.byref(self.m), _p(w), len(w))
 
     def kick(self, c: float):
         self.lib.orc_kick(C.byref(self.m), c)

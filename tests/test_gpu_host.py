@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from cosmolattice_amd.clmi import CLMI_F64, FLD_S, PI_S, Lattice, VV_WEIGHTS
+from cosmolattice_amd.clmi import CLMI_F64, FLD_S, PI_S, PV_WEIGHTS, Lattice, VV_WEIGHTS
 from test_gpu_parity import bench_spec, say, scalar_ics
 
 pytestmark = pytest.mark.gpu
@@ -107,12 +107,14 @@ def oracle_loop(orc, steps, out_every, evolver="LF", obs=None):
             recs.append(orc.hubble_constraint())
         if evolver == "LF":
             orc.lf_evolve(1)
+        elif evolver.startswith("PV"):
+            orc.pv_evolve(PV_WEIGHTS[evolver], 1)
         else:
             orc.vv_evolve(VV_WEIGHTS[evolver], 1)
     return np.array(recs)
 
 
-@pytest.mark.parametrize("evolver", ["LF", "VV4"])
+@pytest.mark.parametrize("evolver", ["LF", "VV4", "PV4"])
 def test_host_program_matches_oracle(gpu, oracle_mod, tmp_path, evolver):
     say(f"case host lphi4 {evolver}")
     spec = bench_spec()

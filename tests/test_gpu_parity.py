@@ -74,6 +74,8 @@ def run_reference_loop(obj, steps, out_every, evolver="LF", ws=(1.0,)):
             recs.append(obj.hubble_constraint())
         if evolver == "LF":
             obj.lf_evolve(1)
+        elif evolver == "PV":
+            obj.pv_evolve(list(ws), 1)
         else:
             obj.vv_evolve(list(ws), 1)
     if evolver == "LF":
@@ -215,6 +217,56 @@ def test_vv2_single_field_fp64_parity(gpu, oracle_mod):
     for k in [(FLD_S, 0), (PI_S, 0)]:
         assert relerr(lat.download(*k, 0), orc.download(*k)) < FP64_TOL
     np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+
+
+@pytest.mark.parametrize("scheme,N", [("PV2", 16), ("PV4", 20)])
+def test_pv_lphi4_fp64_parity(gpu, oracle_mod, scheme, N):
+    """2-field lphi4 with position Verlet, expansion on, 40 steps with Friedmann checks every 10:
+    fields, momenta, background and Friedmann records match the oracle within 1e-10."""
+    say(f"case pv_lphi4_fp64 {scheme} N={N}")
+    ws = oracle_mod.PV_WEIGHTS[scheme]
+    spec = bench_spec()
+    dx, dt = 0.4909, 0.005
+    rng = np.random.default_rng(200 + N)
+    ics = scalar_ics(spec, N, rng)
+    lat = Lattice(spec, N, dx, dt, CLMI_F64, expansion=True)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=True)
+    load_both(lat, orc, ics)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 40, 10, evolver="PV", ws=ws)
+    ro = run_reference_loop(orc, 40, 10, evolver="PV", ws=ws)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        e = relerr(lat.download(*k, 0), orc.download(*k))
+        say(k, e)
+        assert e < FP64_TOL, k
+    bg, bo = lat.background(), orc.background()
+    for key in bo:
+        if bo[key] != 0:
+            assert abs(bg[key] - bo[key]) <= FP64_TOL * abs(bo[key]), key
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+    assert bg["aI"] > 1.0
+
+
+def test_pv_static_background_parity(gpu, oracle_mod):
+    """PV2 without expansion (drift coefficients from aI, no background updates), FP64 and FP32-bulk."""
+    say("case pv_static")
+    spec = bench_spec()
+    N, dx, dt = 16, 0.4909, 0.01
+    ics = scalar_ics(spec, N, np.random.default_rng(5))
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    for (k, i), a in ics.items():
+        orc.upload(k, i, 0, a)
+    orc.pv_evolve([1.0], 30)
+    for prec, tol in ((CLMI_F64, FP64_TOL), (CLMI_F32_BULK, 1e-5)):
+        lat = Lattice(spec, N, dx, dt, prec, expansion=False)
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.pv_evolve([1.0], 30)
+        for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+            e = relerr(lat.download(*k, 0), orc.download(*k))
+            say(prec, k, e)
+            assert e < tol, (prec, k)
+        lat.close()
 
 
 def test_primitives_fp64_parity(gpu, oracle_mod):
@@ -362,6 +414,28 @@ def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod, N):
     np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
     gg, go = lat.gauss_u1(), orc.gauss_u1()
     np.testing.assert_allclose(gg, go, rtol=1e-9)
+
+
+@pytest.mark.parametrize("N", [16, 64])
+def test_pv_lphi4u1_fp64_parity(gpu, oracle_mod, N):
+    """Charged scalar + U(1) with position Verlet (PV2), expansion on, 20 steps with measurements
+    every 10: all ten arrays and the Friedmann records match the oracle within 1e-10."""
+    say(f"case pv_lphi4U1_fp64 N={N}")
+    spec = u1_spec()
+    dx, dt = 0.6, 0.01
+    st = u1_ics(spec, N, np.random.default_rng(37))
+    lat = Lattice(spec, N, dx, dt, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_u1(lat, st)
+    load_u1(orc, st)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 20, 10, evolver="PV")
+    ro = run_reference_loop(orc, 20, 10, evolver="PV")
+    for (k, c) in st:
+        e = relerr(lat.download(k, 0, c), orc.download(k, 0, c))
+        say((k, c), e)
+        assert e < FP64_TOL, (k, c)
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
 
 
 @pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (64, CLMI_F32_BULK), (128, CLMI_F32_BULK), (128, CLMI_F64)])

@@ -126,6 +126,106 @@ def test_free_plane_wave_exact_recurrence(oracle_mod, evolver):
     np.testing.assert_allclose(pi, -A * math.sin(steps * W * dt) * math.sin(W * dt) / dt * shape, atol=1e-12)
 
 
+def test_composition_weights_satisfy_order_conditions(oracle_mod):
+    """Symmetric compositions of a second-order symmetric stage: the stage weights sum to 1, and
+    sum(g^3) = 0 from order 4, sum(g^5) = 0 from order 6 (Yoshida 1990).  The product's velocity- and
+    position-Verlet tables equal the oracle's.
+
+    The order-4 table keeps the reference's second weight, -1.702414403875838200264
+    (positionverletparameters.h:25, velocityverletparameters.h:29), which differs from Yoshida's
+    closed form -2^(1/3) / (2 - 2^(1/3)) = -1.70241438391931... by -2.0e-8: a VV4/PV4 step of the
+    reference advances the fields by (1 - 2.0e-8) dt.  The drop-in reproduces that value."""
+    from cosmolattice_amd.clmi import PV_WEIGHTS, VV_WEIGHTS
+    order = {"PV2": 2, "PV4": 4, "PV6": 6, "PV6_2": 6, "PV8": 8, "PV10": 10}
+    x1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    w4 = oracle_mod.PV_WEIGHTS["PV4"]
+    assert abs(w4[0] - x1) < 1e-15
+    assert abs(w4[1] - (-(2.0 ** (1.0 / 3.0)) * x1) + 1.995652e-8) < 1e-13
+    for name, ws in oracle_mod.PV_WEIGHTS.items():
+        n = len(ws)
+        g = np.array([ws[s if s < n else 2 * n - 2 - s] for s in range(2 * n - 1)])
+        tol = 2.1e-8 if name == "PV4" else 1e-14
+        assert abs(g.sum() - 1.0) < tol, name
+        if order[name] >= 4:
+            assert abs(np.sum(g ** 3)) < (2e-7 if name == "PV4" else 1e-13), name
+        if order[name] >= 6:
+            assert abs(np.sum(g ** 5)) < 1e-13, name
+        assert PV_WEIGHTS[name] == ws
+        assert VV_WEIGHTS["VV" + name[2:]] == ws
+
+
+def test_position_verlet_free_plane_wave_exact_recurrence(oracle_mod):
+    """Position Verlet (drift w dt/2, kick w dt, drift w dt/2) on a free plane wave, static
+    background: phi_n = A cos(n th) with cos(th) = 1 - w^2 dt^2 / 2 as for leapfrog, but the
+    momenta are pi_n = -A w^2 dt sin(n th) / sin(th), which differs from the velocity-Verlet value
+    by the factor 1 / (1 - w^2 dt^2 / 4): a PV step run as a VV step fails this."""
+    N, dx, dt, m2 = 8, 0.5, 0.05, 0.7
+    n = (2, 0, 1)
+    A = 0.3
+    lat = oracle_mod.OracleLattice(free_field(m2), N, dx, dt, expansion=False)
+    f0 = plane_wave(N, n, A)
+    lat.upload(FLD_S, 0, 0, f0)
+    w2 = m2 + (4 / dx ** 2) * sum(math.sin(math.pi * ni / N) ** 2 for ni in n)
+    th = math.acos(1 - w2 * dt * dt / 2)
+    steps = 41
+    lat.pv_evolve([1.0], steps)
+    shape = f0 / A
+    np.testing.assert_allclose(lat.download(FLD_S, 0), A * math.cos(steps * th) * shape, atol=1e-13)
+    np.testing.assert_allclose(lat.download(PI_S, 0), -A * w2 * dt * math.sin(steps * th) / math.sin(th) * shape,
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme", ["PV4", "PV6"])
+def test_position_verlet_time_reversible(oracle_mod, scheme):
+    """A symmetric composition of position-Verlet stages with dt -> -dt retraces its path."""
+    ws = oracle_mod.PV_WEIGHTS[scheme]
+    N, dx, dt = 8, 0.6, 0.02
+    spec = lphi4(100.0)
+    rng = np.random.default_rng(4)
+    phi0 = 1.0 + smooth_random(N, rng)
+    chi0 = smooth_random(N, rng, amp=0.05)
+    pi0 = smooth_random(N, rng, amp=0.1) - 0.5
+    fw = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+    fw.upload(FLD_S, 0, 0, phi0)
+    fw.upload(FLD_S, 1, 0, chi0)
+    fw.upload(PI_S, 0, 0, pi0)
+    fw.pv_evolve(ws, 20)
+    bw = oracle_mod.OracleLattice(spec, N, dx, -dt, expansion=False)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        bw.upload(*k, 0, fw.download(*k))
+    bw.pv_evolve(ws, 20)
+    np.testing.assert_allclose(bw.download(FLD_S, 0), phi0, atol=1e-12)
+    np.testing.assert_allclose(bw.download(FLD_S, 1), chi0, atol=1e-12)
+    np.testing.assert_allclose(bw.download(PI_S, 0), pi0, atol=1e-12)
+    assert np.max(np.abs(fw.download(FLD_S, 0) - phi0)) > 1e-3
+
+
+def test_position_verlet_orders(oracle_mod):
+    """Global error after a fixed time shrinks as dt^p: 2^2 for PV2 and 2^4 for PV4 when dt halves
+    (error measured against a PV4 run at dt/8).  A wrong stage order or weight drops PV4 to second
+    order."""
+    N, dx, T = 8, 0.7, 0.4
+    spec = lphi4(30.0)
+    rng = np.random.default_rng(6)
+    init = [(FLD_S, 0, 1.0 + smooth_random(N, rng)), (FLD_S, 1, smooth_random(N, rng, amp=0.1)),
+            (PI_S, 0, smooth_random(N, rng, amp=0.2))]
+
+    def run(ws, dt):
+        lat = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=False)
+        for k, i, v in init:
+            lat.upload(k, i, 0, v)
+        lat.pv_evolve(ws, int(round(T / dt)))
+        return lat.download(FLD_S, 0)
+
+    pv2, pv4 = oracle_mod.PV_WEIGHTS["PV2"], oracle_mod.PV_WEIGHTS["PV4"]
+    ref = run(pv4, 0.0025)
+    e2 = [np.max(np.abs(run(pv2, dt) - ref)) for dt in (0.04, 0.02)]
+    e4 = [np.max(np.abs(run(pv4, dt) - ref)) for dt in (0.04, 0.02)]
+    assert 3.5 < e2[0] / e2[1] < 4.5
+    assert 13.0 < e4[0] / e4[1] < 19.0
+    assert e4[1] < e2[1] / 20
+
+
 def test_leapfrog_time_reversible(oracle_mod):
     """Synced leapfrog with dt -> -dt retraces its path (static background)."""
     N, dx, dt = 8, 0.6, 0.02
@@ -200,6 +300,8 @@ def reference_run(lat, steps, out_every, evolver="LF", ws=(1.0,)):
             recs.append(lat.hubble_constraint())
         if evolver == "LF":
             lat.lf_evolve(1)
+        elif evolver.startswith("PV"):
+            lat.pv_evolve(list(ws), 1)
         else:
             lat.vv_evolve(list(ws), 1)
     if evolver == "LF":
@@ -222,7 +324,7 @@ def _init_expanding(oracle_mod, spec, N, dx, dt, seed=7):
     return lat
 
 
-@pytest.mark.parametrize("evolver", ["LF", "VV2"])
+@pytest.mark.parametrize("evolver", ["LF", "VV2", "PV2"])
 def test_friedmann_constraint_second_order(oracle_mod, evolver):
     """With self-consistent expansion the first Friedmann equation holds to O(dt^2): the worst
     |rel_diff_friedmann| shrinks ~4x when dt halves.  A wrong sign or factor in the scale-factor
