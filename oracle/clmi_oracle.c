@@ -18,7 +18,10 @@
  * Pinning (tests/test_oracle.py): closed forms and invariants pin the Laplacian, the potentials,
  * kick/drift (exact free-wave recurrence, time reversibility), the scale-factor update and the
  * reductions (O(dt^2) Friedmann and energy constraints), the U(1) sector (gauge invariance, Gauss
- * law, gQ = 0 limit) and the spectrum bins (Parseval, counts, single-mode bin).
+ * law, gQ = 0 limit) and the spectrum bins (Parseval, counts, single-mode bin).  The position
+ * Verlet (orc_pv_evolve) is pinned by its exact free-wave recurrence (momenta differ from the
+ * velocity Verlet's by 1/(1 - w^2 dt^2/4)), time reversibility of PV4/PV6, the dt^2 / dt^4 global
+ * error orders of PV2 / PV4 and the O(dt^2) Friedmann constraint with expansion.
  * Parity unpinned against the reference (TempLat could not be built or imported): the Gauss-ratio
  * denominator of orc_gauss_u1, which follows definitions/gausslaws.h:43-47 as written.  The
  * spectrum prefactors and the GRF random stream live in oracle.py (same status, stated there).
