@@ -101,9 +101,12 @@ def _gxx() -> str:
 
 
 def host_compile_cmd(src: Path, out: Path, model_header: str, model_class: str) -> list:
-    """g++ command for a C++ program over the CosmoInterface mirror and libclmi.so."""
+    """g++ command for a C++ program over the CosmoInterface mirror and libclmi.so.  MODELLABEL is the
+    executable's name, as the reference's CMakeLists.txt:139 defines it (model.name, used in the
+    backup and output file names)."""
     return [_gxx(), "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{HOST}", f"-I{INCLUDE}",
-            f'-DCLMI_MODEL_HEADER="{model_header}"', f"-DCLMI_MODEL_CLASS={model_class}", str(src),
+            f'-DCLMI_MODEL_HEADER="{model_header}"', f"-DCLMI_MODEL_CLASS={model_class}",
+            f"-DMODELLABEL={out.name}", str(src),
             "-o", str(out), f"-L{PKG}", "-lclmi", f"-Wl,-rpath,{PKG}"]
 
 

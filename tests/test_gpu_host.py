@@ -238,8 +238,9 @@ def test_host_backup_and_restart(gpu, tmp_path):
     say("case host restart")
     N, dx, dt = 16, 0.4909, 0.005
     spec = bench_spec()
+    # the backup interval in steps is tBackupFreq / dt truncated (runparameters.h:216): 20.5 dt -> 20
     params = lphi4_params(N, dx, dt, 60, 60, extra="kCutOff = 3.0\nbaseSeed = 5\n") + \
-        f"tBackupFreq = {20 * dt!r}\n"
+        f"tBackupFreq = {20.5 * dt!r}\n"
     full = tmp_path / "full"
     full.mkdir()
     run_program("lphi4", full, params, f"backup_dir={full}/", f"clmi_dump_dir={full}/end")
