@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "clmi_internal.hpp"
 
@@ -449,7 +450,7 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
     RowsPlan rp;
     for (int xc : {32, 16, 8, 4})
-      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, true, rows_shape(g.flags), &rp)) {
+      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, 1, rows_shape(g.flags), &rp)) {
         c->rowk = true;
         c->rplan = rp;
         break;
@@ -1676,11 +1677,15 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   TiledPlan best_plan = c->tplan;
   RowsPlan best_rplan = c->rplan;
   int rc = CLMI_OK;
-  for (int nt = 0; nt < 2 && rc == CLMI_OK; ++nt)
+  // row kernel: memory modes 0 (plain), 1 (NT stores), 3 (+ NT momentum loads), 7 (+ NT field loads);
+  // tiled kernel: NT stores off / on
+  const std::vector<int> modes = c->rowk ? std::vector<int>{0, 1, 3, 7} : std::vector<int>{0, 1};
+  for (int nt : modes) {
+    if (rc != CLMI_OK) break;
     for (int xc : {4, 8, 16, 32, 64}) {
       TiledPlan p;
       RowsPlan rp;
-      if (c->rowk ? !scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, nt != 0, rows_shape(c->cfg.flags), &rp)
+      if (c->rowk ? !scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, nt, rows_shape(c->cfg.flags), &rp)
                   : !scalar_tiled_plan(c->N, c->cfg.n_scalars, xc, nt != 0, &p))
         continue;
       auto launch = [&]() {
@@ -1699,13 +1704,15 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
         break;
       }
       const double us = 1e3 * ms / reps;
-      rep += "XC=" + std::to_string(xc) + " NT=" + std::to_string(nt) + " " + std::to_string(us) + "\n";
+      rep += "XC=" + std::to_string(xc) + (c->rowk ? " MM=" : " NT=") + std::to_string(nt) + " " +
+             std::to_string(us) + "\n";
       if (us < best) {
         best = us;
         best_plan = p;
         best_rplan = rp;
       }
     }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CLMI_OK) return rc;
@@ -1713,7 +1720,7 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
     c->rplan = best_rplan;
     rep = "# kernel k_scalar_rows VZ=" + std::to_string(best_rplan.VZ) + " WPR=" + std::to_string(best_rplan.WPR) +
           " TY=" + std::to_string(best_rplan.TY) + " XC=" + std::to_string(best_rplan.XC) +
-          " NT=" + std::to_string((int)best_rplan.NT) + "\n" + rep;
+          " MM=" + std::to_string(best_rplan.MM) + "\n" + rep;
   } else {
     c->tplan = best_plan;
     rep = "# kernel k_scalar_tiled XC=" + std::to_string(best_plan.XC) + " NT=" + std::to_string((int)best_plan.NT) +

@@ -228,10 +228,10 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p);
 struct RowsPlan {
   int VZ = 0, WPR = 0, TY = 0;  // sites per lane, wavefronts per z-row, rows per workgroup
   int XC = 0, G = 0;
-  bool NT = false;
+  int MM = 1;  // memory mode: bit 0 non-temporal stores, bit 1 NT momentum loads, bit 2 NT field loads
   int blocks = 0, rows = 0;
 };
-bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, int shape, RowsPlan *p);
+bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPlan *p);
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,

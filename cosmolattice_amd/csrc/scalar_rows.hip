@@ -11,7 +11,7 @@ hipError_t launch_scalar_rows_f64(int nf, bool drift, bool kick, bool red, const
 // Row-kernel shapes by (N, precision): VZ sites per lane, WPR wavefronts per row, TY rows per
 // workgroup.  The shape fixes the summation order; the x-chunk XC and store mode are tuned at run
 // time without changing results.
-bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, int shape, RowsPlan *p)
+bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPlan *p)
 {
   if (nf != 2 && nf != 1) return false;
   int vz = 0, wpr = 0, ty = 0;
@@ -24,12 +24,13 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, bool nt, int shape, RowsP
   else return false;
   const int G = 8;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
+  if (mm != 0 && mm != 1 && mm != 3 && mm != 7) return false;
   p->VZ = vz;
   p->WPR = wpr;
   p->TY = ty;
   p->XC = XC;
   p->G = G;
-  p->NT = nt;
+  p->MM = mm;
   p->blocks = (N / ty) * (N / XC);
   p->rows = (N / ty) * (N / G);
   return true;

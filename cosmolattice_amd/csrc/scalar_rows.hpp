@@ -8,7 +8,9 @@
 // TY rows of the current plane plus one halo row above and below, and z-neighbours are the lane's
 // own registers except at its two edges, which are read from the same LDS row.  Because rows are
 // complete there is no z-halo; the y-halo is two full rows per plane, the x-halo two planes per
-// chunk.  Outputs go out with non-temporal stores when NT.
+// chunk.  Memory mode MM: bit 0 non-temporal stores, bit 1 non-temporal loads of the momenta (read
+// exactly once), bit 2 non-temporal loads of the workgroup's own field rows (the y-halo rows, which
+// the neighbouring tile also reads, always use ordinary loads).
 //
 // Volume sums are written per (group of G planes, TY-row y-tile), independent of XC.
 #pragma once
@@ -99,26 +101,33 @@ __device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
 template <typename S> using gptr = __attribute__((address_space(1))) S *;
 template <typename S> using cgptr = const __attribute__((address_space(1))) S *;
 
-// VZ consecutive elements with vector accesses (the site index of element 0 is a multiple of VZ).
-template <int VZ, typename S>
+// VZ consecutive elements with vector accesses (the site index of element 0 is a multiple of VZ);
+// NTL: non-temporal loads.
+template <bool NTL, typename Q>
+__device__ __forceinline__ Q ldq(const __attribute__((address_space(1))) Q *p)
+{
+  if constexpr (NTL) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int VZ, bool NTL = false, typename S>
 __device__ __forceinline__ void vload(const S *pg, uint32_t s, S (&o)[VZ])
 {
   const cgptr<S> p = (cgptr<S>)pg;
   if constexpr (VZ == 4 && sizeof(S) == 4) {
     typedef S v4 __attribute__((ext_vector_type(4)));
-    const v4 v = *(cgptr<v4>)(p + s);
+    const v4 v = ldq<NTL, v4>((cgptr<v4>)(p + s));
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
   } else if constexpr (VZ % 2 == 0) {
     typedef S v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int k = 0; k < VZ; k += 2) {
-      const v2 v = *(cgptr<v2>)(p + s + k);
+      const v2 v = ldq<NTL, v2>((cgptr<v2>)(p + s + k));
       o[k] = v.x;
       o[k + 1] = v.y;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < VZ; ++k) o[k] = p[s + k];
+    for (int k = 0; k < VZ; ++k) o[k] = ldq<NTL, S>(p + s + k);
   }
 }
 template <int VZ, bool NT, typename S>
@@ -158,11 +167,11 @@ struct Tr<false> {
   struct Raw {
     double f[VZ], p[VZ];
   };
-  template <int VZ>
+  template <int VZ, bool NTF = false, bool NTP = false>
   __device__ static __forceinline__ void load(const ScalarPassArgs &a, int f, uint32_t s, bool need_p, Raw<VZ> &r)
   {
-    vload<VZ>((const double *)a.phi_in[f], s, r.f);
-    if (need_p) vload<VZ>((const double *)a.pi_in[f], s, r.p);
+    vload<VZ, NTF>((const double *)a.phi_in[f], s, r.f);
+    if (need_p) vload<VZ, NTP>((const double *)a.pi_in[f], s, r.p);
     else
 #pragma unroll
       for (int k = 0; k < VZ; ++k) r.p[k] = 0.0;
@@ -183,12 +192,12 @@ struct Tr<true> {
   struct Raw {
     float h[VZ], l[VZ], p[VZ];
   };
-  template <int VZ>
+  template <int VZ, bool NTF = false, bool NTP = false>
   __device__ static __forceinline__ void load(const ScalarPassArgs &a, int f, uint32_t s, bool need_p, Raw<VZ> &r)
   {
-    vload<VZ>((const float *)a.phi_in[f], s, r.h);
-    vload<VZ>(a.philo_in[f], s, r.l);
-    if (need_p) vload<VZ>((const float *)a.pi_in[f], s, r.p);
+    vload<VZ, NTF>((const float *)a.phi_in[f], s, r.h);
+    vload<VZ, NTF>(a.philo_in[f], s, r.l);
+    if (need_p) vload<VZ, NTP>((const float *)a.pi_in[f], s, r.p);
     else
 #pragma unroll
       for (int k = 0; k < VZ; ++k) r.p[k] = 0.0f;
@@ -227,13 +236,14 @@ template <int NF, bool F32, int VZ, int WPR, int TY>
 using RowsTile = typename Tr<F32>::V[2][NF][TY + 2][64 * VZ * WPR];
 
 // One workgroup's work: workgroup b of the nwg that cover this lattice.
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, bool NT>
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM>
 __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC, const int G, const int b,
                                           const int nwg, RowsTile<NF, F32, VZ, WPR, TY> &ldsbuf)
 {
   using T = Tr<F32>;
   using V = typename T::V;
   using Raw = typename T::template Raw<VZ>;
+  constexpr bool NT = (MM & 1) != 0, NTP = (MM & 2) != 0, NTF = (MM & 4) != 0;
   constexpr int NW = WPR * TY;  // wavefronts per workgroup
   constexpr int N = 64 * VZ * WPR;
 
@@ -274,17 +284,17 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     Raw q;
-    T::load(a, f, rowbase(x0 - 1, y) + zb, DRIFT, q);
+    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0 - 1, y) + zb, DRIFT, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) prev[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
-    T::load(a, f, rowbase(x0, y) + zb, true, q);
+    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0, y) + zb, true, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
       cur[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
       pic[f][k] = q.p[k];
       lds[f][row + 1][zb + k] = cur[f][k];
     }
-    T::load(a, f, rowbase(x0 + 1, y) + zb, true, q);
+    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0 + 1, y) + zb, true, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
       nxt[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
@@ -306,7 +316,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
     if (more) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        T::load(a, f, rowbase(x + 2, y) + zb, true, pf[f]);
+        T::template load<VZ, NTF, NTP>(a, f, rowbase(x + 2, y) + zb, true, pf[f]);
         if (halo) T::load(a, f, rowbase(x + 1, hy) + zb, DRIFT, hq[f]);
       }
     }
@@ -443,14 +453,14 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   }
 }
 
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, bool NT>
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM>
 __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
 {
   __shared__ RowsTile<NF, F32, VZ, WPR, TY> ldsbuf;
-  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, NT>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
+  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, MM>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
 }
 
-template <int NF, bool F32, int VZ, int WPR, int TY, bool NT>
+template <int NF, bool F32, int VZ, int WPR, int TY, int MM>
 hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
                          hipStream_t s)
 {
@@ -458,7 +468,7 @@ hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &
   const dim3 blk(64 * WPR * TY);
 #define CLMI_SR(D, K, R)                                                                                       \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                             \
-    hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, NT>), dim3(blocks), blk, 0, s, a, XC, G); \
+    hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, MM>), dim3(blocks), blk, 0, s, a, XC, G); \
     break;
   switch (code) {
     CLMI_SR(true, true, true)

@@ -9,8 +9,13 @@ hipError_t launch_scalar_rows_f32(int nf, bool drift, bool kick, bool red, const
   const bool f32 = true;
 #define CLMI_SHAPE(NF_, F32_, VZ_, WPR_, TY_)                                                          \
   if (nf == NF_ && f32 == F32_ && p.VZ == VZ_ && p.WPR == WPR_ && p.TY == TY_)                         \
-    return p.NT ? dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s) \
-                : dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    switch (p.MM) {                                                                                    \
+      case 0: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+      case 1: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+      case 3: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 3>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+      case 7: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 7>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+      default: return hipErrorInvalidValue;                                                            \
+    }
   CLMI_SHAPE(2, true, 4, 1, 8)
   CLMI_SHAPE(2, true, 2, 2, 8)
   CLMI_SHAPE(2, true, 2, 1, 8)
