@@ -25,7 +25,9 @@ HIP_SOURCES = [
     "scalar_tiled.hip",
     "scalar_rows.hip",
     "scalar_rows_f32.hip",
+    "scalar_rows_f32_1f.hip",
     "scalar_rows_f64.hip",
+    "scalar_rows_f64_1f.hip",
     "scalar_rows_batch.hip",
     "gauge_pass.hip",
     "gauge_tiled.hip",

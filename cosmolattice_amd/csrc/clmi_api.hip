@@ -112,6 +112,9 @@ int fail(clmi_ctx *c, int code, const std::string &msg)
 size_t bytes_field(bool f32) { return f32 ? 4 : 8; }
 
 // Row-pass shape selector from clmi_config.flags (bit 0: VZ = 4 rows, bit 1: 4-row tiles).
+// Memory mode of the full-row pass when neither the configuration nor clmi_autotune chooses one.
+constexpr int kDefaultMemMode = 7;
+
 int rows_shape(int flags)
 {
   return ((flags & CLMI_FLAG_SHAPE_ALT) ? 1 : 0) | ((flags & CLMI_FLAG_SHAPE_TY4) ? 2 : 0);
@@ -449,12 +452,18 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   c->blocks_g = gauge_pass_blocks(c->V);
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
     RowsPlan rp;
+    const int mm_req = (g.flags >> 8) & 15;
     for (int xc : {32, 16, 8, 4})
-      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, 1, rows_shape(g.flags), &rp)) {
+      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, mm_req ? mm_req : kDefaultMemMode,
+                           rows_shape(g.flags), &rp)) {
         c->rowk = true;
         c->rplan = rp;
         break;
       }
+    if (mm_req && !c->rowk) {
+      fail(c, CLMI_E_INVALID, "memory mode " + std::to_string(mm_req) + " is not available for this lattice");
+      return bail(CLMI_E_INVALID);
+    }
     TiledPlan p;
     if (!c->rowk)
       for (int xc : {16, 8, 4, 32, 64})
@@ -1677,9 +1686,9 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   TiledPlan best_plan = c->tplan;
   RowsPlan best_rplan = c->rplan;
   int rc = CLMI_OK;
-  // row kernel: memory modes 0 (plain), 1 (NT stores), 3 (+ NT momentum loads), 7 (+ NT field loads);
+  // row kernel: memory modes 1 (NT stores), 3 (+ NT momentum loads), 7 (+ NT loads of own field rows);
   // tiled kernel: NT stores off / on
-  const std::vector<int> modes = c->rowk ? std::vector<int>{0, 1, 3, 7} : std::vector<int>{0, 1};
+  const std::vector<int> modes = c->rowk ? std::vector<int>{1, 3, 7} : std::vector<int>{0, 1};
   for (int nt : modes) {
     if (rc != CLMI_OK) break;
     for (int xc : {4, 8, 16, 32, 64}) {

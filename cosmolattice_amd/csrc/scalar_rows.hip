@@ -3,10 +3,14 @@
 
 namespace clmi {
 
-hipError_t launch_scalar_rows_f32(int nf, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                                  const RowsPlan &p, hipStream_t s);
-hipError_t launch_scalar_rows_f64(int nf, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                                  const RowsPlan &p, hipStream_t s);
+hipError_t launch_scalar_rows_f32_nf2(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
+                                      hipStream_t s);
+hipError_t launch_scalar_rows_f32_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
+                                      hipStream_t s);
+hipError_t launch_scalar_rows_f64_nf2(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
+                                      hipStream_t s);
+hipError_t launch_scalar_rows_f64_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
+                                      hipStream_t s);
 
 // Row-kernel shapes by (N, precision): VZ sites per lane, WPR wavefronts per row, TY rows per
 // workgroup.  The shape fixes the summation order; the x-chunk XC and store mode are tuned at run
@@ -24,7 +28,7 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
   else return false;
   const int G = 8;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
-  if (mm != 0 && mm != 1 && mm != 3 && mm != 7) return false;
+  if (mm != 1 && mm != 3 && mm != 7) return false;
   p->VZ = vz;
   p->WPR = wpr;
   p->TY = ty;
@@ -39,7 +43,11 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s)
 {
-  return f32 ? launch_scalar_rows_f32(nf, drift, kick, red, a, p, s) : launch_scalar_rows_f64(nf, drift, kick, red, a, p, s);
+  if (nf == 2)
+    return f32 ? launch_scalar_rows_f32_nf2(drift, kick, red, a, p, s) : launch_scalar_rows_f64_nf2(drift, kick, red, a, p, s);
+  if (nf == 1)
+    return f32 ? launch_scalar_rows_f32_nf1(drift, kick, red, a, p, s) : launch_scalar_rows_f64_nf1(drift, kick, red, a, p, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace clmi

@@ -49,6 +49,10 @@ extern "C" {
 /* clmi_config.flags: 4-row workgroup tiles of the full-row lattice pass (N = 256, two fields),
  * combinable with CLMI_FLAG_SHAPE_ALT.  Same results as above. */
 #define CLMI_FLAG_SHAPE_TY4 4
+/* clmi_config.flags: memory mode m of the full-row lattice pass (0 = the library's default, 7; 1 =
+ * non-temporal stores, 3 = also non-temporal loads of the momenta, 7 = also of the workgroup's own
+ * field rows).  Results do not depend on it; clmi_autotune chooses one. */
+#define CLMI_FLAG_MEMMODE(m) ((m) << 8)
 
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
 #define CLMI_FLD_S 0
