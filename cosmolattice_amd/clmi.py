@@ -30,6 +30,12 @@ CLMI_F32_BULK = 1
 CLMI_MAX_SCALARS = 4
 CLMI_MAX_TERMS = 8
 CLMI_FLAG_NO_TILED = 1
+CLMI_FLAG_SHAPE_ALT = 2
+CLMI_FLAG_SHAPE_TY4 = 4
+
+
+def CLMI_FLAG_MEMMODE(m: int) -> int:
+    return m << 8
 
 FLD_S, PI_S, FLD_CS, PI_CS, FLD_U1, PI_U1 = 0, 1, 2, 3, 6, 7
 
