@@ -122,6 +122,27 @@ def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
     }
 
 
+TRAFFIC_FILE = "profiles/k_scalar_rows_lphi4_2f_256_traffic.md"
+
+
+def measured_traffic(kernel: str, f32: bool):
+    """HBM bytes per launch of the fused pass with the chosen plan, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the same workload (calibrated against a copy kernel), or None
+    when that summary has no row for this kernel instantiation."""
+    import re
+    m = re.search(r"VZ=(\d+) WPR=(\d+) TY=(\d+) XC=\d+ MM=(\d+)", kernel)
+    path = Path(__file__).resolve().parent / TRAFFIC_FILE
+    if not m or not path.exists():
+        return None
+    vz, wpr, ty, mm = m.groups()
+    name = f"`k_scalar_rows<2, {'true' if f32 else 'false'}, {vz}, {wpr}, {ty}, true, true, true, {mm}>`"
+    for ln in path.read_text().splitlines():
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == name:
+            return (float(cells[2]) + float(cells[3])) * 2 ** 20
+    return None
+
+
 def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000):
     """Charged scalar + U(1) (lphi4U1, e^2/lambda = 10, program units lambda = 1) on 256^3 with the
     Gauss-consistent vacuum initial conditions: site-updates/s and HBM fraction of the fused pass in
@@ -277,11 +298,13 @@ def main():
     acc["t_end"] = (args.warmup + args.steps) * dt
 
     r32, r64 = res["f32_bulk"], res["f64"]
-    def roof(r):
+    def roof(r, f32):
         avg_s = r["kernel_ms"] / 1e3 / max(r["launches"], 1)
         achieved = r["bytes_per_launch"] / avg_s / 1e9
+        traffic = measured_traffic(r["kernel"], f32)
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": TRAFFIC_FILE if traffic is not None else None,
                 "kernel": r["kernel"] + " (fused drift+kick+reductions)",
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"],
                 "autotune_us": r["autotune_us"]}
@@ -317,12 +340,12 @@ def main():
                                "self-consistent expansion)", "N": N, "kIR": kIR, "dt": dt, "q": q,
                    "mode": "FP32-bulk: fields f32 hi/lo, momenta f32, reductions + scale factor f64",
                    "fStar_over_MPl": spec.fStar / spec.MPl},
-        "roofline": roof(r32),
+        "roofline": roof(r32, True),
         "cpu_baseline": cpu,
         "fp64": {"value": r64["site_updates_per_s"], "unit": "site-updates/s", "ms_per_step": r64["ms_per_step"],
-                 "roofline": roof(r64)},
+                 "roofline": roof(r64, False)},
         "f32_over_f64": r32["site_updates_per_s"] / r64["site_updates_per_s"],
-        "hbm_frac_f32": roof(r32)["frac"],
+        "hbm_frac_f32": roof(r32, True)["frac"],
         "accuracy": acc,
         "ensemble": ens,
         "u1": u1,
