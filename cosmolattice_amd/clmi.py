@@ -32,6 +32,7 @@ CLMI_MAX_TERMS = 8
 CLMI_FLAG_NO_TILED = 1
 CLMI_FLAG_SHAPE_ALT = 2
 CLMI_FLAG_SHAPE_TY4 = 4
+CLMI_FLAG_GAUGE_TY8 = 16
 
 
 def CLMI_FLAG_MEMMODE(m: int) -> int:

@@ -49,7 +49,8 @@ struct clmi_ctx {
   TiledPlan tplan;
   bool rowk = false;   // scalar passes use the full-row kernel with plan rplan
   RowsPlan rplan;
-  bool gtiled = false; // charged scalar + U(1) passes use the tiled kernel (x-chunk gXC)
+  bool gtiled = false; // charged scalar + U(1) passes use the tiled kernel (gTY-row tiles, x-chunk gXC)
+  int gTY = 4;
   int gXC = 0;
   size_t partial_rows = 0;
 
@@ -113,7 +114,7 @@ size_t bytes_field(bool f32) { return f32 ? 4 : 8; }
 
 // Row-pass shape selector from clmi_config.flags (bit 0: VZ = 4 rows, bit 1: 4-row tiles).
 // Memory mode of the full-row pass when neither the configuration nor clmi_autotune chooses one.
-constexpr int kDefaultMemMode = 7;
+constexpr int kDefaultMemMode = 15;
 
 int rows_shape(int flags)
 {
@@ -265,7 +266,7 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     a.partials = c->partials + (size_t)r * kNumRed;
     a.poly = c->cpoly;
     if (c->gtiled)
-      HIPCHK(c, launch_gauge_tiled(c->f32, drift, kick, red, a, c->blocks_g, c->gXC, c->stream));
+      HIPCHK(c, launch_gauge_tiled(c->f32, c->gTY, drift, kick, red, a, c->blocks_g, c->gXC, c->stream));
     else
       HIPCHK(c, launch_gauge_pass(c->f32, drift, kick, red, a, c->blocks_g, c->stream));
     if (red) r += c->blocks_g;
@@ -473,8 +474,10 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
           break;
         }
     int gxc = 0, gblocks = 0;
-    if (g.n_cscalars > 0 && gauge_tiled_plan(g.N, kMaxPassBlocks, &gxc, &gblocks)) {
+    const int gty = (g.flags & CLMI_FLAG_GAUGE_TY8) ? 8 : 4;
+    if (g.n_cscalars > 0 && gauge_tiled_plan(g.N, gty, kMaxPassBlocks, &gxc, &gblocks)) {
       c->gtiled = true;
+      c->gTY = gty;
       c->gXC = gxc;
       c->blocks_g = gblocks;
     }
@@ -1686,9 +1689,9 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   TiledPlan best_plan = c->tplan;
   RowsPlan best_rplan = c->rplan;
   int rc = CLMI_OK;
-  // row kernel: memory modes 1 (NT stores), 3 (+ NT momentum loads), 7 (+ NT loads of own field rows);
-  // tiled kernel: NT stores off / on
-  const std::vector<int> modes = c->rowk ? std::vector<int>{1, 3, 7} : std::vector<int>{0, 1};
+  // row kernel: memory modes 1 (NT stores), 3 (+ NT momentum loads), 7 (+ NT loads of own field rows),
+  // 15 (as 7, but ordinary loads for the tile's edge rows); tiled kernel: NT stores off / on
+  const std::vector<int> modes = c->rowk ? std::vector<int>{1, 3, 7, 15} : std::vector<int>{0, 1};
   for (int nt : modes) {
     if (rc != CLMI_OK) break;
     for (int xc : {4, 8, 16, 32, 64}) {

@@ -148,10 +148,11 @@ struct GaugePassArgs {
 
 // Deterministic block reduction: wavefront shuffles, then a fixed-order sum over the block's
 // wavefronts; thread k < K writes slot k of the block's row.
-template <int K>
-__device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
+// Sum of v over a block of NTB threads, stored by threads 0..K-1 into row[0..K).
+template <int K, int NTB>
+__device__ __forceinline__ void block_reduce_store_nt(double (&v)[K], double *row)
 {
-  __shared__ double sh[kBlock / 64][K];
+  __shared__ double sh[NTB / 64][K];
 #pragma unroll
   for (int k = 0; k < K; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
@@ -163,15 +164,21 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
   if (threadIdx.x < K) {
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) s += sh[w][threadIdx.x];
+    for (int w = 0; w < NTB / 64; ++w) s += sh[w][threadIdx.x];
     row[threadIdx.x] = s;
   }
 }
 
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
+{
+  block_reduce_store_nt<K, kBlock>(v, row);
+}
+
 // ---- tiled charged scalar + U(1) pass (gauge_tiled.hip) ----
-bool gauge_tiled_plan(int N, int max_blocks, int *XC, int *blocks);
-hipError_t launch_gauge_tiled(bool f32, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC,
-                              hipStream_t s);
+bool gauge_tiled_plan(int N, int ty, int max_blocks, int *XC, int *blocks);
+hipError_t launch_gauge_tiled(bool f32, int ty, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks,
+                              int XC, hipStream_t s);
 
 // ---- ensembles: several lattices in one launch (scalar_rows_batch.hip, background.hip) ----
 struct RowsMember {   // one lattice's fused pass inside a batched launch (device-resident table)

@@ -1,12 +1,13 @@
 // Charged scalar + U(1) lattice pass, x-marching with LDS plane tiles (N a multiple of 64).
 //
-// A workgroup of four wavefronts owns a tile of 4 y-rows x 64 z-sites (one site per lane) and marches
-// along x over XC planes.  The drifted values of all five field components (phi_0, phi_1, A_x, A_y,
+// A workgroup of TY wavefronts (TY = 4 or 8) owns a tile of TY y-rows x 64 z-sites (one site per
+// lane) and marches along x over XC planes.  The drifted values of all five field components (phi_0, phi_1, A_x, A_y,
 // A_z) of planes x-1, x and x+1, each with a one-site y/z halo including corners, sit in a
 // three-slot LDS ring, so every neighbour the covariant Laplacian, the gauge Laplacian, the mixed
 // derivatives d-_j d+_i A_j (the diagonal sites x+i-j) and the links need is read from LDS; each
 // field value is loaded from memory and drifted once per tile instead of once per use.  The raw
-// values of plane x+2 are loaded while plane x is computed.  The per-site arithmetic is
+// values of plane x+2 are loaded while plane x is computed; the momenta of a thread's own site stay
+// in registers from that load until the kick uses them.  The per-site arithmetic is
 // gauge_site.hpp's, so fields are bit-identical to the one-site-per-thread pass; volume sums are
 // per workgroup (one partial row each).
 #include "gauge_site.hpp"
@@ -15,7 +16,6 @@ namespace clmi {
 namespace {
 
 constexpr int GTZ = 64;  // z sites per tile (one per lane)
-constexpr int GTY = 4;   // y rows per tile (one per wavefront)
 constexpr int GNC = 5;   // components: phi_0, phi_1, A_x, A_y, A_z
 
 // Raw (not yet drifted) values of one site's five components.
@@ -64,8 +64,8 @@ template <> struct GRaw<true> {
   }
 };
 
-template <bool F32, bool DRIFT, bool KICK, bool RED>
-__global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, const int XC)
+template <bool F32, bool DRIFT, bool KICK, bool RED, int GTY>
+__global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a, const int XC)
 {
   using G = GVal<F32>;
   using V = typename G::V;
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, cons
     return (uint32_t)wrap(xx) * NN + (uint32_t)wrap(yy) * N + (uint32_t)wrap(zz);
   };
 
-  // halo cell of this thread (140 of 256 threads): rows y0-1 and y0+4 with corners, columns z0-1, z0+64
+  // halo cell of this thread (2 * 66 + 2 * GTY of 64 * GTY threads): rows y0-1 and y0+GTY with corners,
+  // columns z0-1, z0+64
   int hly = -1, hlz = 0, hy = 0, hz = 0;
   if (t < GTZ + 2) {
     hly = 0; hlz = t; hy = y0 - 1; hz = z0 - 1 + t;
@@ -123,12 +124,24 @@ __global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, cons
   };
   auto slot_of = [&](int xx) -> int { return (xx - x0 + 1) % 3; };
 
+  // own-site momenta of planes x and x + 1 (read with the fields when drifting, else here)
+  P pc[GNC], pn[GNC];
+  auto keep_p = [&](const GRaw<F32> &r, P (&dst)[GNC]) {
+    for (int q = 0; q < GNC; ++q) dst[q] = r.p[q];
+  };
+  auto load_p = [&](uint32_t s, P (&dst)[GNC]) {
+    for (int q = 0; q < 2; ++q) dst[q] = ((const P *)a.pi_in[q])[s];
+    for (int d = 0; d < 3; ++d) dst[2 + d] = u1 ? ((const P *)a.piA_in[d])[s] : P(0);
+  };
+
   // prologue: planes x0-1, x0, x0+1
   for (int xx = x0 - 1; xx <= x0 + 1; ++xx) {
     GRaw<F32> own, hal;
     own.load(a, site(xx, y, z), u1, DRIFT);
     if (halo) hal.load(a, site(xx, hy, hz), u1, DRIFT);
     put(slot_of(xx), own, hal, halo);
+    if (KICK && xx == x0) { if (DRIFT) keep_p(own, pc); else load_p(site(xx, y, z), pc); }
+    if (KICK && xx == x0 + 1) { if (DRIFT) keep_p(own, pn); else load_p(site(xx, y, z), pn); }
   }
   __syncthreads();
 
@@ -182,9 +195,15 @@ __global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, cons
         for (int d = 0; d < 3; ++d) G::store(a.A_out[d], a.Alo_out[d], s, g.Ac[d]);
     }
     P pin[2], pAin[3] = {0, 0, 0}, pout[2], pAout[3];
-    for (int q = 0; q < 2; ++q) pin[q] = ((const P *)a.pi_in[q])[s];
-    if (u1)
-      for (int d = 0; d < 3; ++d) pAin[d] = ((const P *)a.piA_in[d])[s];
+    if (KICK) {
+      for (int q = 0; q < 2; ++q) pin[q] = pc[q];
+      if (u1)
+        for (int d = 0; d < 3; ++d) pAin[d] = pc[2 + d];
+    } else {
+      for (int q = 0; q < 2; ++q) pin[q] = ((const P *)a.pi_in[q])[s];
+      if (u1)
+        for (int d = 0; d < 3; ++d) pAin[d] = ((const P *)a.piA_in[d])[s];
+    }
     if constexpr (F32)
       gauge_site_f32<KICK, RED>(k, a.poly, g, pin, pAin, pout, pAout, acc);
     else
@@ -195,6 +214,11 @@ __global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, cons
         for (int d = 0; d < 3; ++d) ((P *)a.piA_out[d])[s] = pAout[d];
     }
     if (more) {
+      if (KICK) {
+        for (int q = 0; q < GNC; ++q) pc[q] = pn[q];
+        if (DRIFT) keep_p(pown, pn);
+        else if (i + 2 < XC) load_p(site(x + 2, y, z), pn);
+      }
       __syncthreads();  // every wavefront is done with slot sm (plane x - 1)
       put(sm, pown, phal, halo);  // plane x + 2 takes its slot
       __syncthreads();
@@ -202,17 +226,17 @@ __global__ __launch_bounds__(256) void k_gauge_tiled(const GaugePassArgs a, cons
   }
   if (RED) {
     double v[kNumRed] = {0, 0, acc[2], acc[0], acc[1], acc[3], acc[4], 0};
-    block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
+    block_reduce_store_nt<kNumRed, 64 * GTY>(v, a.partials + (size_t)blockIdx.x * kNumRed);
   }
 }
 
-template <bool F32>
+template <bool F32, int GTY>
 hipError_t dispatch_tiled(bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC, hipStream_t s)
 {
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
 #define CLMI_GT(D, K, R)                                                                              \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                    \
-    hipLaunchKernelGGL((k_gauge_tiled<F32, D, K, R>), dim3(blocks), dim3(256), 0, s, a, XC);          \
+    hipLaunchKernelGGL((k_gauge_tiled<F32, D, K, R, GTY>), dim3(blocks), dim3(64 * GTY), 0, s, a, XC); \
     break;
   switch (code) {
     CLMI_GT(true, true, true)
@@ -231,12 +255,12 @@ hipError_t dispatch_tiled(bool drift, bool kick, bool red, const GaugePassArgs &
 
 }  // namespace
 
-// Plan of the tiled pass: x-chunk XC (a divisor of N, at least 8) keeping the workgroup count within
-// max_blocks; false when N does not fit the tile.
-bool gauge_tiled_plan(int N, int max_blocks, int *XC, int *blocks)
+// Plan of the tiled pass with TY-row tiles (4 or 8): x-chunk XC (a divisor of N, at least 8) keeping
+// the workgroup count within max_blocks; false when N does not fit the tile.
+bool gauge_tiled_plan(int N, int ty, int max_blocks, int *XC, int *blocks)
 {
-  if (N % GTZ != 0 || N % GTY != 0) return false;
-  const long tiles = (long)(N / GTY) * (N / GTZ);
+  if ((ty != 4 && ty != 8) || N % GTZ != 0 || N % ty != 0) return false;
+  const long tiles = (long)(N / ty) * (N / GTZ);
   for (int xc = 8; xc <= N; xc *= 2) {
     if (N % xc != 0) continue;
     const long nb = tiles * (N / xc);
@@ -249,11 +273,16 @@ bool gauge_tiled_plan(int N, int max_blocks, int *XC, int *blocks)
   return false;
 }
 
-hipError_t launch_gauge_tiled(bool f32, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC,
-                              hipStream_t s)
+hipError_t launch_gauge_tiled(bool f32, int ty, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks,
+                              int XC, hipStream_t s)
 {
-  return f32 ? dispatch_tiled<true>(drift, kick, red, a, blocks, XC, s)
-             : dispatch_tiled<false>(drift, kick, red, a, blocks, XC, s);
+  if (ty == 8)
+    return f32 ? dispatch_tiled<true, 8>(drift, kick, red, a, blocks, XC, s)
+               : dispatch_tiled<false, 8>(drift, kick, red, a, blocks, XC, s);
+  if (ty == 4)
+    return f32 ? dispatch_tiled<true, 4>(drift, kick, red, a, blocks, XC, s)
+               : dispatch_tiled<false, 4>(drift, kick, red, a, blocks, XC, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace clmi

@@ -28,7 +28,7 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
   else return false;
   const int G = 8;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
-  if (mm != 1 && mm != 3 && mm != 7) return false;
+  if (mm != 1 && mm != 3 && mm != 7 && mm != 15) return false;
   p->VZ = vz;
   p->WPR = wpr;
   p->TY = ty;

@@ -10,7 +10,8 @@
 // complete there is no z-halo; the y-halo is two full rows per plane, the x-halo two planes per
 // chunk.  Memory mode MM: bit 0 non-temporal stores, bit 1 non-temporal loads of the momenta (read
 // exactly once), bit 2 non-temporal loads of the workgroup's own field rows (the y-halo rows, which
-// the neighbouring tile also reads, always use ordinary loads).
+// the neighbouring tile also reads, always use ordinary loads), bit 3 (with bit 2) ordinary loads for
+// the tile's first and last rows, which are the neighbouring tiles' halo rows.
 //
 // Volume sums are written per (group of G planes, TY-row y-tile), independent of XC.
 #pragma once
@@ -243,7 +244,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   using T = Tr<F32>;
   using V = typename T::V;
   using Raw = typename T::template Raw<VZ>;
-  constexpr bool NT = (MM & 1) != 0, NTP = (MM & 2) != 0, NTF = (MM & 4) != 0;
+  constexpr bool NT = (MM & 1) != 0, NTP = (MM & 2) != 0, NTF = (MM & 4) != 0, EDGE = (MM & 8) != 0;
   constexpr int NW = WPR * TY;  // wavefronts per workgroup
   constexpr int N = 64 * VZ * WPR;
 
@@ -274,6 +275,15 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   const int hy = row == 0 ? y0 - 1 : y0 + TY;
   const int hl = row == 0 ? 0 : TY + 1;
 
+  // own-row loads; with EDGE the first and last tile rows (wave-uniform) use ordinary field loads
+  const bool edge_row = EDGE && (row == 0 || row == TY - 1);
+  auto own_load = [&](int f, uint32_t s, bool need_p, Raw &q) {
+    if (edge_row)
+      T::template load<VZ, false, NTP>(a, f, s, need_p, q);
+    else
+      T::template load<VZ, NTF, NTP>(a, f, s, need_p, q);
+  };
+
   V prev[NF][VZ], cur[NF][VZ], nxt[NF][VZ];
   typename T::P pic[NF][VZ], pin[NF][VZ];
   Raw pf[NF], hq[NF];
@@ -284,17 +294,17 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     Raw q;
-    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0 - 1, y) + zb, DRIFT, q);
+    own_load(f, rowbase(x0 - 1, y) + zb, DRIFT, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) prev[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
-    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0, y) + zb, true, q);
+    own_load(f, rowbase(x0, y) + zb, true, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
       cur[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
       pic[f][k] = q.p[k];
       lds[f][row + 1][zb + k] = cur[f][k];
     }
-    T::template load<VZ, NTF, NTP>(a, f, rowbase(x0 + 1, y) + zb, true, q);
+    own_load(f, rowbase(x0 + 1, y) + zb, true, q);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
       nxt[f][k] = T::template value<DRIFT>(q, k, fs, a.cdt, cf);
@@ -316,7 +326,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
     if (more) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        T::template load<VZ, NTF, NTP>(a, f, rowbase(x + 2, y) + zb, true, pf[f]);
+        own_load(f, rowbase(x + 2, y) + zb, true, pf[f]);
         if (halo) T::load(a, f, rowbase(x + 1, hy) + zb, DRIFT, hq[f]);
       }
     }

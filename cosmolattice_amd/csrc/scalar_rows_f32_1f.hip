@@ -11,6 +11,7 @@ hipError_t launch_scalar_rows_f32_nf1(bool drift, bool kick, bool red, const Sca
       case 1: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
       case 7: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 7>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
       case 3: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 3>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+      case 15: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 15>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
       default: return hipErrorInvalidValue;                                                            \
     }
   CLMI_SHAPE(1, true, 4, 1, 8)

@@ -49,9 +49,13 @@ extern "C" {
 /* clmi_config.flags: 4-row workgroup tiles of the full-row lattice pass (N = 256, two fields),
  * combinable with CLMI_FLAG_SHAPE_ALT.  Same results as above. */
 #define CLMI_FLAG_SHAPE_TY4 4
-/* clmi_config.flags: memory mode m of the full-row lattice pass (0 = the library's default, 7; 1 =
+/* clmi_config.flags: 8-row tiles (512-thread workgroups) in the tiled charged-scalar + U(1) pass
+ * instead of 4-row tiles.  Fields bit-identical; volume sums differ in summation order only. */
+#define CLMI_FLAG_GAUGE_TY8 16
+/* clmi_config.flags: memory mode m of the full-row lattice pass (0 = the library's default, 15; 1 =
  * non-temporal stores, 3 = also non-temporal loads of the momenta, 7 = also of the workgroup's own
- * field rows).  Results do not depend on it; clmi_autotune chooses one. */
+ * field rows, 15 = as 7 but ordinary loads for the tile's first and last rows).  Results do not
+ * depend on it; clmi_autotune chooses one. */
 #define CLMI_FLAG_MEMMODE(m) ((m) << 8)
 
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */

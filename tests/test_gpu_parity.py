@@ -121,7 +121,8 @@ def test_lf_lphi4_fp64_parity_100_steps(gpu, oracle_mod, N):
     (256, CLMI_F32_BULK, 1 << 8), (256, CLMI_F64, 1 << 8), (256, CLMI_F32_BULK, 3 << 8),
     (256, CLMI_F64, 3 << 8), (128, CLMI_F32_BULK, 1 << 8), (192, CLMI_F64, 1 << 8),
     (64, CLMI_F32_BULK, 3 << 8), (256, CLMI_F32_BULK, (1 << 8) | 4), (256, CLMI_F64, (1 << 8) | 2),
-    (256, CLMI_F32_BULK, 7 << 8), (256, CLMI_F64, 7 << 8), (192, CLMI_F32_BULK, 7 << 8)])
+    (256, CLMI_F32_BULK, 7 << 8), (256, CLMI_F64, 7 << 8), (192, CLMI_F32_BULK, 7 << 8),
+    (256, CLMI_F32_BULK, 15 << 8), (256, CLMI_F64, 15 << 8), (128, CLMI_F64, 15 << 8)])
 def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
     """The full-row pass (N = 64, 128, 192, 256, all workgroup shapes and memory modes) and the
     tiled x-marching pass (other multiples of 64) give fields and momenta bit-identical to the generic
@@ -442,17 +443,21 @@ def test_pv_lphi4u1_fp64_parity(gpu, oracle_mod, N):
     np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
 
 
-@pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (64, CLMI_F32_BULK), (128, CLMI_F32_BULK), (128, CLMI_F64)])
-def test_tiled_gauge_pass_bit_identical_to_generic(gpu, N, prec):
+@pytest.mark.parametrize("N,prec,ty8", [(64, CLMI_F64, False), (64, CLMI_F32_BULK, False),
+                                        (128, CLMI_F32_BULK, False), (128, CLMI_F64, False),
+                                        (64, CLMI_F64, True), (128, CLMI_F32_BULK, True)])
+def test_tiled_gauge_pass_bit_identical_to_generic(gpu, N, prec, ty8):
     """The tiled charged-scalar + U(1) pass gives the same fields, bit for bit, as the
     one-site-per-thread pass (flag CLMI_FLAG_NO_TILED); volume averages agree to summation order."""
-    say(f"case gauge tiled N={N} prec={prec}")
+    from cosmolattice_amd.clmi import CLMI_FLAG_GAUGE_TY8, CLMI_FLAG_NO_TILED
+
+    say(f"case gauge tiled N={N} prec={prec} ty8={ty8}")
     spec = u1_spec()
     dx, dt = 0.6, 0.01
     rng = np.random.default_rng(35)
     st = u1_ics(spec, N, rng)
     res = []
-    for flags in (0, 1):
+    for flags in (CLMI_FLAG_GAUGE_TY8 if ty8 else 0, CLMI_FLAG_NO_TILED):
         lat = Lattice(spec, N, dx, dt, prec, expansion=False, flags=flags)
         load_u1(lat, st)
         lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=0.0)
