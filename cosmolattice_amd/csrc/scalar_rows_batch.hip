@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_scalar_rows_batch(const RowsM
   const RowsMember &mm = m[i];
   const int lb = b - mm.start;
   if (lb >= mm.blocks) return;
-  rows_body<NF, F32, VZ, 1, 8, D, K, R, 1>(mm.a, mm.XC, mm.G, lb, mm.blocks, lds);
+  rows_body<NF, F32, VZ, 1, 8, D, K, R, 15>(mm.a, mm.XC, mm.G, lb, mm.blocks, lds);
 }
 
 template <int NF, bool F32, int VZ>
