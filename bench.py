@@ -122,6 +122,114 @@ def cpu_baseline(lat64, spec, N, dx, dt, budget_s):
     }
 
 
+def c1_bench(clmi, steps=1000, N=64, kIR=0.1, dt=0.01, seed=11):
+    """Single-field lphi4 64^3 with velocity Verlet (VV2), FP64, self-consistent expansion, seeded vacuum
+    fluctuations: the whole run on the GPU, and the same run on the FP64 oracle on this host's cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    from cosmolattice_amd.clmi import FLD_S, PI_S
+
+    spec = clmi.lphi4(0.0, n_fields=1)
+    log(f"[bench] c1: lphi4 1 field {N}^3 VV2, {steps} steps, GPU and oracle")
+    lat, dx = make_lattice(clmi, spec, N, kIR, dt, clmi.CLMI_F64, seed)
+    ref = orc.OracleLattice(spec, N, dx, dt, expansion=True)
+    ref.upload(FLD_S, 0, 0, lat.download(FLD_S, 0, 0))
+    ref.upload(PI_S, 0, 0, lat.download(PI_S, 0, 0))
+    ref.set_background(**lat.background())
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.vv_evolve([1.0], steps)
+    lat.synchronize()
+    gpu_s = time.perf_counter() - t0
+    fr_gpu = lat.hubble_constraint()[0]
+    t0 = time.perf_counter()
+    ref.vv_evolve([1.0], steps)
+    cpu_s = time.perf_counter() - t0
+    fr_cpu = ref.hubble_constraint()[0]
+    err = max(float(np.max(np.abs(lat.download(k, 0, 0) - ref.download(k, 0))) / np.max(np.abs(ref.download(k, 0))))
+              for k in (FLD_S, PI_S))
+    out = {"config": f"lphi4 1 field {N}^3, kIR={kIR}, VV2, dt={dt}, {steps} steps, FP64, expansion",
+           "gpu_site_updates_per_s": N ** 3 * steps / gpu_s, "gpu_seconds": gpu_s,
+           "cpu_site_updates_per_s": N ** 3 * steps / cpu_s, "cpu_seconds": cpu_s, "cpu_cores": orc.num_threads(),
+           "max_rel_diff_gpu_vs_oracle": err, "friedmann_gpu": fr_gpu, "friedmann_oracle": fr_cpu,
+           "aI_end": lat.background()["aI"]}
+    lat.close()
+    return out
+
+
+def adjoint_bench(clmi, N=128, kIR=0.05, dt=0.005, steps=2000, seed=5000, n_modes=256):
+    """Adjoint sensitivities of the 2-field lphi4 128^3 run (2000 leapfrog steps): loss = final energy
+    fraction of chi + chi power in k in [0.5, 1.5]; dL/dq and dL/d(initial phi mode) for 256 seeded
+    modes from the reverse-mode leapfrog (sqrt(T) checkpoints), FP64 and FP32-bulk forward, against
+    central differences of FP64 forward runs.  Over 2000 steps the loss is nonlinear enough in q that a
+    relative step of 1e-4 is outside the linear range, so the differences are taken at relative steps
+    1e-4, 1e-6 and 1e-8 and all are reported."""
+    import torch
+    from cosmolattice_amd.adjoint import chi_energy_fraction_and_band, leapfrog_run, mode_gradients
+    from cosmolattice_amd.clmi import FLD_S, PI_S
+
+    spec = clmi.lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+    log(f"[bench] adjoint: {N}^3, {steps} steps")
+    lat, dx = make_lattice(clmi, spec, N, kIR, dt, clmi.CLMI_F64, seed)
+    phi0 = np.stack([lat.download(FLD_S, f, 0) for f in range(2)])
+    pi0 = np.stack([lat.download(PI_S, f, 0) for f in range(2)])
+    lat.close()
+    coeffs = [c for c, _ in spec.terms]
+    band = (0.5, 1.5)
+
+    def run(prec, cvals, p0=phi0, grads=True):
+        c = torch.tensor(cvals, dtype=torch.float64, requires_grad=grads)
+        p = torch.from_numpy(p0.copy()).requires_grad_(grads)
+        q = torch.from_numpy(pi0.copy()).requires_grad_(grads)
+        t0 = time.perf_counter()
+        phiT, piT, bgT = leapfrog_run(c, p, q, spec, N, dx, dt, steps, prec, True, 0)
+        L = chi_energy_fraction_and_band(phiT, piT, bgT, c, spec, dx, kIR, band)
+        if not grads:
+            return float(L.detach()), time.perf_counter() - t0
+        L.backward()
+        return float(L.detach()), time.perf_counter() - t0, c.grad.numpy(), p.grad.numpy()
+
+    L32, t32, gc32, gp32 = run(clmi.CLMI_F32_BULK, coeffs)
+    L64, t64, gc64, gp64 = run(clmi.CLMI_F64, coeffs)
+    # q enters as the coefficient q/2 of phi^2 chi^2
+    dq32, dq64 = 0.5 * gc32[1], 0.5 * gc64[1]
+    fd_q = {}
+    for rel in (1e-4, 1e-6, 1e-8):
+        h = rel * abs(coeffs[1])
+        cp, cm = list(coeffs), list(coeffs)
+        cp[1] += h
+        cm[1] -= h
+        fd_q[f"{rel:g}"] = 0.5 * (run(clmi.CLMI_F64, cp, grads=False)[0] - run(clmi.CLMI_F64, cm, grads=False)[0]) / (2 * h)
+    rng = np.random.default_rng(seed)
+    kc = N / 4  # modes below the initial-condition cutoff |n| kIR < N kIR / 4
+    modes = []
+    while len(modes) < n_modes:
+        n = rng.integers(-int(kc), int(kc) + 1, size=3)
+        if 0 < n[2] < N // 2 and np.linalg.norm(n) < kc and not any(np.array_equal(n, m) for m in modes):
+            modes.append(n)
+    m32, m64 = mode_gradients(gp32[0], modes), mode_gradients(gp64[0], modes)
+    # two of the modes (real parts) by central differences at relative step 1e-8
+    x = np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    fd_modes = []
+    for j in range(2):
+        shape = 2.0 / N ** 3 * np.cos(2 * np.pi * (modes[j][0] * X + modes[j][1] * Y + modes[j][2] * Z) / N)
+        eps = 1e-8 * np.max(np.abs(phi0[0])) / np.max(np.abs(shape))
+        pp, pm = phi0.copy(), phi0.copy()
+        pp[0] += eps * shape
+        pm[0] -= eps * shape
+        fd = (run(clmi.CLMI_F64, coeffs, pp, grads=False)[0] - run(clmi.CLMI_F64, coeffs, pm, grads=False)[0]) / (2 * eps)
+        fd_modes.append({"mode": [int(v) for v in modes[j]], "adjoint_f64": float(m64[j][0]), "central_difference_f64": fd})
+    return {"config": f"lphi4 2 fields {N}^3, q=100, kIR={kIR}, dt={dt}, {steps} leapfrog steps, expansion; "
+                      f"loss = E_chi/E_tot + chi power in k in [{band[0]}, {band[1]}]",
+            "forward_plus_adjoint_seconds": {"f32_bulk": t32, "f64": t64},
+            "loss": {"f32_bulk": L32, "f64": L64},
+            "dL_dq": {"f32_bulk": dq32, "f64": dq64, "central_difference_f64_by_rel_step": fd_q,
+                      "rel_err_f64_vs_smallest_step": abs(dq64 - fd_q["1e-08"]) / abs(fd_q["1e-08"])},
+            "mode_gradients": {"modes": n_modes, "central_difference_checks": fd_modes,
+                               "max_rel_diff_f32_vs_f64": float(np.max(np.abs(m32 - m64)) / np.max(np.abs(m64)))}}
+
+
 TRAFFIC_FILE = "profiles/k_scalar_rows_lphi4_2f_256_traffic.md"
 
 
@@ -249,6 +357,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--u1-steps", type=int, default=500)
     ap.add_argument("--no-u1", action="store_true")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 VV2 GPU-vs-oracle run")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -322,6 +433,8 @@ def main():
         lat.close()
     ens = None if args.no_ensemble else ensemble_bench(clmi, spec, args.ensemble_steps)
     u1 = None if args.no_u1 else u1_bench(clmi, args.u1_steps)
+    c1 = None if args.no_c1 else c1_bench(clmi)
+    adj = None if args.no_adjoint else adjoint_bench(clmi, steps=args.adjoint_steps)
 
     line = {
         "metric": METRIC,
@@ -349,6 +462,8 @@ def main():
         "accuracy": acc,
         "ensemble": ens,
         "u1": u1,
+        "c1_vv2_64": c1,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

@@ -724,3 +724,49 @@ def test_adjoint_restatement_matches_central_differences(oracle_mod, expansion):
             vals.append(_fd_loss(oracle_mod, dataclasses.replace(spec, terms=terms), N, dx, dt, phi0, pi0, nsteps,
                                  expansion, weights))
         assert gc[t] == pytest.approx((vals[0] - vals[1]) / (2 * h), rel=1e-6)
+
+
+def test_adjoint_restatement_long_run_needs_small_difference_steps(oracle_mod):
+    """Over 2000 leapfrog steps of the q = 100 model the final state depends on the coupling
+    nonlinearly enough that a central difference with relative step 1e-4 misses the derivative by
+    a large factor, while steps of 1e-7 and 1e-8 converge to the reverse-mode value (to 1e-4, the
+    bar of the sensitivity runs).
+    The adjoint is the exact derivative of the discrete map; long-run gradient checks must use
+    small steps."""
+    import dataclasses
+
+    spec = lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+    N, dt, kIR, nsteps = 8, 0.005, 0.4, 2000
+    dx = 2 * math.pi / kIR / N
+    rng = np.random.default_rng(3)
+    phi0 = np.stack([spec.fld0[f] + smooth_random(N, rng, amp=0.05) + 0.01 * rng.normal(size=(N,) * 3)
+                     for f in range(2)])
+    pi0 = np.stack([spec.pi0[f] + smooth_random(N, rng, amp=0.05) + 0.01 * rng.normal(size=(N,) * 3)
+                    for f in range(2)])
+    wf = [rng.normal(size=(N,) * 3) / N ** 3 for _ in range(2)]
+    wp = [rng.normal(size=(N,) * 3) / N ** 3 for _ in range(2)]
+    wa = (0.7, -1.3)
+    o = oracle_mod.OracleLattice(spec, N, dx, dt, expansion=True)
+    for f in range(2):
+        o.upload(FLD_S, f, 0, phi0[f])
+        o.upload(PI_S, f, 0, pi0[f])
+    _, _, rhs = o.hubble_constraint()
+    o.set_background(aDotI=math.sqrt(rhs))
+    o.lf_init()
+    o.lf_evolve(nsteps)
+    o.lf_sync()
+    seed_f = [2 * wf[f] * o.download(FLD_S, f) for f in range(2)]
+    _, _, gc = oracle_mod.adjoint_lf(spec, N, dx, dt, phi0, pi0, nsteps, seed_f, wp, wa, True)
+
+    def fd(rel):
+        h = rel * abs(spec.terms[1][0])
+        vals = []
+        for sgn in (1, -1):
+            terms = [(c + (sgn * h if i == 1 else 0.0), e) for i, (c, e) in enumerate(spec.terms)]
+            vals.append(_fd_loss(oracle_mod, dataclasses.replace(spec, terms=terms), N, dx, dt, phi0, pi0, nsteps,
+                                 True, (wf, wp, wa)))
+        return (vals[0] - vals[1]) / (2 * h)
+
+    assert gc[1] == pytest.approx(fd(1e-7), rel=1e-4)
+    assert gc[1] == pytest.approx(fd(1e-8), rel=1e-4)
+    assert abs(fd(1e-4) - gc[1]) > 1e-2 * abs(gc[1])
