@@ -126,6 +126,12 @@ __device__ __forceinline__ void vload(const S *pg, uint32_t s, S (&o)[VZ])
       o[k] = v.x;
       o[k + 1] = v.y;
     }
+  } else if constexpr (VZ == 3 && sizeof(S) == 4) {
+    // one 12-byte access (global_load_dwordx3), aligned to the element only (s is a multiple of 3).
+    // Only for 4-byte elements: a 3-element vector of doubles is accessed as 32 bytes.
+    typedef S v3 __attribute__((ext_vector_type(3), aligned(sizeof(S))));
+    const v3 v = ldq<NTL, v3>((cgptr<v3>)(p + s));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z;
   } else {
 #pragma unroll
     for (int k = 0; k < VZ; ++k) o[k] = ldq<NTL, S>(p + s + k);
@@ -149,6 +155,12 @@ __device__ __forceinline__ void vstore(S *pg, uint32_t s, const S (&o)[VZ])
       v.y = o[k + 1];
       if constexpr (NT) __builtin_nontemporal_store(v, (gptr<v2>)(p + s + k)); else *(gptr<v2>)(p + s + k) = v;
     }
+  } else if constexpr (VZ == 3 && sizeof(S) == 4) {
+    // one 12-byte access (global_store_dwordx3); see vload
+    typedef S v3 __attribute__((ext_vector_type(3), aligned(sizeof(S))));
+    v3 v;
+    v.x = o[0]; v.y = o[1]; v.z = o[2];
+    if constexpr (NT) __builtin_nontemporal_store(v, (gptr<v3>)(p + s)); else *(gptr<v3>)(p + s) = v;
   } else {
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
