@@ -693,8 +693,10 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
       if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
       CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex,
                       {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
-      CHK(pass_update(c, true, false, ex, dt * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
-      CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex, {{OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
+      // drift and second half-kick in one pass: the drift factor (a_SI) and the kick factors (a_I) are both
+      // set by the update above, and the pass's sums are the drifted fields' and the kicked momenta's
+      CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
+                      {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
     }
   return CLMI_OK;
 }

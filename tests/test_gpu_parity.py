@@ -422,6 +422,30 @@ def test_lf_lphi4u1_fp64_parity(gpu, oracle_mod, N):
 
 
 @pytest.mark.parametrize("N", [16, 64])
+def test_vv4_lphi4u1_fp64_parity(gpu, oracle_mod, N):
+    """Charged scalar + U(1) with velocity Verlet (VV4: three stages, each a kick pass and a fused
+    drift + kick pass), expansion on, 10 steps with measurements every 5: all ten arrays and the
+    Friedmann records match the oracle within 1e-10."""
+    say(f"case vv4_lphi4U1_fp64 N={N}")
+    ws = [1.351207191959657771818, -1.702414403875838200264]
+    spec = u1_spec()
+    dx, dt = 0.6, 0.01
+    st = u1_ics(spec, N, np.random.default_rng(41))
+    lat = Lattice(spec, N, dx, dt, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_u1(lat, st)
+    load_u1(orc, st)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 10, 5, evolver="VV", ws=ws)
+    ro = run_reference_loop(orc, 10, 5, evolver="VV", ws=ws)
+    for (k, c) in st:
+        e = relerr(lat.download(k, 0, c), orc.download(k, 0, c))
+        say((k, c), e)
+        assert e < FP64_TOL, (k, c)
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+
+
+@pytest.mark.parametrize("N", [16, 64])
 def test_pv_lphi4u1_fp64_parity(gpu, oracle_mod, N):
     """Charged scalar + U(1) with position Verlet (PV2), expansion on, 20 steps with measurements
     every 10: all ten arrays and the Friedmann records match the oracle within 1e-10."""
