@@ -484,6 +484,7 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   }
   size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
   if (c->tiled && (size_t)c->tplan.rows + c->blocks_g + 16 > rows) rows = (size_t)c->tplan.rows + c->blocks_g + 16;
+  if (c->rowk && (size_t)c->rplan.rows + 16 > rows) rows = (size_t)c->rplan.rows + 16;
   c->partial_rows = rows;
   if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||

@@ -25,6 +25,8 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
   else if (N == 128) { vz = 2; wpr = 1; ty = 8; }
   else if (N == 64) { vz = 1; wpr = 1; ty = 8; }
   else if (N == 192) { vz = 3; wpr = 1; ty = 8; }
+  // 1024 threads; 8-row tiles would need 2048.  FP64 at 512^3 runs faster in the tiled pass.
+  else if (N == 512 && f32) { vz = 2; wpr = 4; ty = 4; }
   else return false;
   const int G = 8;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;

@@ -19,6 +19,7 @@ hipError_t launch_scalar_rows_f32_nf2(bool drift, bool kick, bool red, const Sca
   CLMI_SHAPE(2, true, 2, 1, 8)
   CLMI_SHAPE(2, true, 1, 1, 8)
   CLMI_SHAPE(2, true, 3, 1, 8)
+  CLMI_SHAPE(2, true, 2, 4, 4)
   CLMI_SHAPE(2, true, 2, 2, 4)
   CLMI_SHAPE(2, true, 4, 1, 4)
 #undef CLMI_SHAPE

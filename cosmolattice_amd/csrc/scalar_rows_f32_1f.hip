@@ -19,6 +19,7 @@ hipError_t launch_scalar_rows_f32_nf1(bool drift, bool kick, bool red, const Sca
   CLMI_SHAPE(1, true, 2, 1, 8)
   CLMI_SHAPE(1, true, 1, 1, 8)
   CLMI_SHAPE(1, true, 3, 1, 8)
+  CLMI_SHAPE(1, true, 2, 4, 4)
 #undef CLMI_SHAPE
   return hipErrorInvalidValue;
 }

@@ -19,6 +19,7 @@ hipError_t launch_scalar_rows_f64_nf2(bool drift, bool kick, bool red, const Sca
   CLMI_SHAPE(2, false, 2, 1, 8)
   CLMI_SHAPE(2, false, 1, 1, 8)
   CLMI_SHAPE(2, false, 3, 1, 8)
+  CLMI_SHAPE(2, false, 2, 4, 4)
   CLMI_SHAPE(2, false, 2, 2, 4)
   CLMI_SHAPE(2, false, 4, 1, 4)
 #undef CLMI_SHAPE

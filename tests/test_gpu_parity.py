@@ -122,9 +122,10 @@ def test_lf_lphi4_fp64_parity_100_steps(gpu, oracle_mod, N):
     (256, CLMI_F64, 3 << 8), (128, CLMI_F32_BULK, 1 << 8), (192, CLMI_F64, 1 << 8),
     (64, CLMI_F32_BULK, 3 << 8), (256, CLMI_F32_BULK, (1 << 8) | 4), (256, CLMI_F64, (1 << 8) | 2),
     (256, CLMI_F32_BULK, 7 << 8), (256, CLMI_F64, 7 << 8), (192, CLMI_F32_BULK, 7 << 8),
-    (256, CLMI_F32_BULK, 15 << 8), (256, CLMI_F64, 15 << 8), (128, CLMI_F64, 15 << 8)])
+    (256, CLMI_F32_BULK, 15 << 8), (256, CLMI_F64, 15 << 8), (128, CLMI_F64, 15 << 8),
+    (512, CLMI_F32_BULK, 0)])
 def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
-    """The full-row pass (N = 64, 128, 192, 256, all workgroup shapes and memory modes) and the
+    """The full-row pass (N = 64, 128, 192, 256, 512, all workgroup shapes and memory modes) and the
     tiled x-marching pass (other multiples of 64) give fields and momenta bit-identical to the generic
     one-site-per-thread pass after 40 leapfrog steps on a static background (same per-site
     arithmetic); their volume sums differ only in summation order."""
