@@ -293,15 +293,24 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
     sites = sum(N ** 3 for N in sizes)
     # default: one group per size (three parallel graph branches); split: groups capped at 128 MiB
     timings = {}
+    tune = {}
     for label, gb in (("split_groups", 128 << 20), ("default", 0)):
         ens = Ensemble(members, group_bytes=gb)
+        if label == "default":
+            tune["ensemble"] = ens.autotune(3)
+            log(f"[bench] ensemble autotune\n{tune['ensemble']}")
         ens.lf_evolve(2)  # first step of each member alone, then the graphs are captured
         t0 = time.perf_counter()
         ens.lf_evolve(steps)
         timings[label] = (time.perf_counter() - t0, ens.groups)
         ens.close()
     t_ens, ngroups = timings["default"]
+    # one lattice at a time, each with its own single-lattice autotuned plan
+    for N in (64, 128, 192):
+        m = next(m for m in members if m.N == N)
+        tune[f"single_{N}"] = m.autotune(2).splitlines()[0]
     for m in members:
+        m.autotune(2)
         m.synchronize()
     t0 = time.perf_counter()
     for m in members:
@@ -337,6 +346,7 @@ def ensemble_bench(clmi, spec, steps, kIR=0.05, dt=0.005, seed=7000):
         "split_groups_site_updates_per_s": sites * steps / timings["split_groups"][0],
         "split_groups": timings["split_groups"][1],
         "one_at_a_time_site_updates_per_s": sites * steps / t_one,
+        "autotune": tune,
         "speedup": t_one / t_ens,
         "ms_per_step": 1e3 * t_ens / steps,
         "mean_chi_spectrum_rel_err_vs_fp64": {"mean": float(band.mean()), "max": float(band.max())},

@@ -130,6 +130,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
                                       C.POINTER(C.c_double), C.POINTER(Background)]),
         "clmi_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
         "clmi_ensemble_groups": (C.c_int, [P]),
+        "clmi_ensemble_autotune": (C.c_int, [P, C.c_int, C.c_char_p, C.c_size_t]),
         "clmi_ensemble_destroy": (None, [P]),
         "clmi_ensemble_last_error": (C.c_char_p, [P]),
         "clmi_ensemble_lf_evolve": (C.c_int, [P, C.c_int]),
@@ -525,6 +526,15 @@ class Ensemble:
     @property
     def groups(self) -> int:
         return self.lib.clmi_ensemble_groups(self._h)
+
+    def autotune(self, reps: int = 3) -> str:
+        """Pick each group's x-chunk by timing its batched pass on the live data (no state change)."""
+        buf = C.create_string_buffer(4096)
+        rc = self.lib.clmi_ensemble_autotune(self._h, reps, buf, len(buf))
+        if rc != CLMI_OK:
+            raise ClmiError(f"clmi_ensemble_autotune failed ({rc}): "
+                            f"{self.lib.clmi_ensemble_last_error(self._h).decode()}")
+        return buf.value.decode()
 
     def close(self):
         if getattr(self, "_h", None):

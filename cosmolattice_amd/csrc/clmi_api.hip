@@ -1628,6 +1628,76 @@ const char *clmi_ensemble_last_error(const clmi_ensemble *e) { return e ? e->err
 
 int clmi_ensemble_groups(const clmi_ensemble *e) { return e ? (int)e->groups.size() : 0; }
 
+int clmi_ensemble_autotune(clmi_ensemble *e, int reps, char *report, size_t report_len)
+{
+  if (!e) return efail(nullptr, CLMI_E_INVALID, "null ensemble");
+  if (report && report_len) report[0] = '\0';
+  if (reps < 1) reps = 1;
+  EHIP(e, hipSetDevice(e->device));
+  for (clmi_ctx *c : e->m) EHIP(e, hipStreamSynchronize(c->stream));
+  CHK(ens_tables(e));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  EHIP(e, hipEventCreate(&t0));
+  EHIP(e, hipEventCreate(&t1));
+  std::string rep;
+  int rc = CLMI_OK;
+  // Each group's batched fused pass from the current buffers into the spare ones (parity-0 table),
+  // timed per x-chunk; no parity flip and no background update, so the members' state is unchanged.
+  for (size_t gi = 0; gi < e->groups.size() && rc == CLMI_OK; ++gi) {
+    const clmi_ensemble::Group g0 = e->groups[gi];
+    double best = 1e300;
+    std::vector<RowsPlan> best_plans;
+    std::string line = "# group VZ=" + std::to_string(g0.vz) + " members=" + std::to_string(g0.count);
+    for (int xc : {8, 16, 32, 64}) {
+      std::vector<RowsPlan> plans;
+      bool ok = true;
+      for (int i = g0.first; i < g0.first + g0.count && ok; ++i) {
+        const clmi_ctx *c = e->order[i];
+        RowsPlan rp;
+        ok = scalar_rows_plan(c->N, c->cfg.n_scalars, c->f32, xc, c->rplan.MM, rows_shape(c->cfg.flags), &rp);
+        plans.push_back(rp);
+      }
+      if (!ok) continue;
+      for (int i = g0.first; i < g0.first + g0.count; ++i) e->order[i]->rplan = plans[i - g0.first];
+      if ((rc = ens_tables(e)) != CLMI_OK) break;
+      const clmi_ensemble::Group &g = e->groups[gi];
+      auto launch = [&]() {
+        return launch_scalar_rows_batch(e->nf, e->f32, g.vz, true, true, e->ex, e->d_rows[0] + g.first, g.count,
+                                        g.blocks, e->stream);
+      };
+      hipError_t he = launch();
+      if (he == hipSuccess) he = hipEventRecord(t0, e->stream);
+      for (int r = 0; r < reps && he == hipSuccess; ++r) he = launch();
+      if (he == hipSuccess) he = hipEventRecord(t1, e->stream);
+      if (he == hipSuccess) he = hipEventSynchronize(t1);
+      float ms = 0.0f;
+      if (he == hipSuccess) he = hipEventElapsedTime(&ms, t0, t1);
+      if (he != hipSuccess) {
+        rc = efail(e, CLMI_E_HIP, std::string("ensemble autotune: ") + hipGetErrorString(he));
+        break;
+      }
+      const double us = 1e3 * ms / reps;
+      line += " XC=" + std::to_string(xc) + ":" + std::to_string(us);
+      if (us < best) {
+        best = us;
+        best_plans = plans;
+      }
+    }
+    if (rc == CLMI_OK && !best_plans.empty())
+      for (int i = g0.first; i < g0.first + g0.count; ++i) e->order[i]->rplan = best_plans[i - g0.first];
+    rep += line + " -> XC=" + std::to_string(best_plans.empty() ? 0 : best_plans[0].XC) + "\n";
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (rc != CLMI_OK) return rc;
+  CHK(ens_tables(e));
+  if (report && report_len) {
+    std::strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = '\0';
+  }
+  return CLMI_OK;
+}
+
 int clmi_ensemble_lf_evolve(clmi_ensemble *e, int nsteps)
 {
   if (!e) return efail(nullptr, CLMI_E_INVALID, "null ensemble");

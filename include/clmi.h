@@ -244,6 +244,11 @@ typedef struct clmi_ensemble clmi_ensemble;
 int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, clmi_ensemble **out);
 /* Number of member groups (formed at the first clmi_ensemble_lf_evolve; 0 before). */
 int clmi_ensemble_groups(const clmi_ensemble *ens);
+/* Autotuner of the batched launches: for every group, times the batched fused pass at each x-chunk
+ * on the members' live data (into their spare buffers, without an update, so no state changes) and
+ * keeps the fastest.  Results do not depend on the choice.  A one-line summary per group goes into
+ * report (may be NULL).  No counterpart in the reference: its ensembles are separate runs. */
+int clmi_ensemble_autotune(clmi_ensemble *ens, int reps, char *report, size_t report_len);
 void clmi_ensemble_destroy(clmi_ensemble *ens);
 const char *clmi_ensemble_last_error(const clmi_ensemble *ens);
 /* nsteps x LeapFrog::evolve on every member (evolvers/leapfrog.h:42-89), as clmi_lf_evolve. */
