@@ -22,6 +22,7 @@
 #include <utility>
 #include <vector>
 
+#include "CosmoInterface/couplingsmanager.h"
 #include "CosmoInterface/runparameters.h"
 #include "CosmoInterface/symbolic.h"
 #include "clmi.h"
@@ -57,13 +58,6 @@ namespace Constants {
 template <typename T> constexpr T reducedMPlanck = T(2.435e18);
 }
 
-// Gauge couplings of the complex scalars (couplingsmanager.h:70-94): NG gauge fields, ON switches the
-// coupling on.
-template <size_t NCS, size_t NG, bool ON> struct CouplingsManager {
-  static constexpr size_t nGauge = NG;
-  static constexpr bool active = ON;
-  static constexpr size_t howManyCouples() { return ON ? NCS * NG : 0; }
-};
 
 struct DefaultModelPars {
   static constexpr size_t NScalars = 0;
@@ -74,7 +68,7 @@ struct DefaultModelPars {
   static constexpr bool DefectsModel = false;
   static constexpr size_t NPotTerms = 1;
   using FloatType = double;
-  typedef CouplingsManager<0, 0, false> CsU1Couplings;
+  typedef CouplingsManager<0, 0> CsU1Couplings;
 };
 
 // Lattice size holder passed to model constructors (the reference's MemoryToolBox carries the
@@ -147,9 +141,11 @@ public:
   {
     // complexscalarbase.h:71-73
     if constexpr (NCs > 0 && NU1 > 0 && CsU1Couplings::active) {
-      const auto g = parser.get<T, CsU1Couplings::nGauge>("gU1s", T(1));
+      const auto g = parser.get<T, (size_t)CsU1Couplings::nGauge>("gU1s", T(1));
       const auto q = parser.get<T, CsU1Couplings::howManyCouples()>("CSU1_charges", T(1));
-      gQ = q[0] * g[0];
+      typename CsU1Couplings::template Container<T> cm;
+      cm.setEffectiveCharges(std::vector<T>(q.begin(), q.end()), std::vector<T>(g.begin(), g.end()));
+      gQ = cm(Tag<0>{}, Tag<0>{});
     }
   }
   AbstractModel(const AbstractModel &) = delete;

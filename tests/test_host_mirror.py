@@ -167,3 +167,37 @@ def test_missing_parameter_is_reported(tmp_path):
 def test_model_programs_built():
     for name in ("lphi4", "lphi4_1f", "lphi4U1"):
         assert (HOST / "_bin" / name).exists(), f"run the build first: {name} main program missing"
+
+
+def test_couplings_manager_matches_reference_kat(tmp_path):
+    """The mirror's CouplingsManager against the known answers of the reference's own test
+    (tests/CosmoInterface/couplingsmanager.cpp:21-58, fixture tests/golden/couplingsmanager_kat.json):
+    the compile-time coupling pattern, g*Q consumed in pattern order over coupled pairs only, the
+    couplings g, and the two count errors."""
+    import json
+
+    from cosmolattice_amd._build import HOST as host_dir
+
+    kat = json.loads((ROOT / "tests" / "golden" / "couplingsmanager_kat.json").read_text())
+    exe = tmp_path / "couplings_kat"
+    r = subprocess.run(["g++", "-std=c++20", "-O1", "-Wall", f"-I{host_dir}", f"-I{ROOT / 'include'}",
+                        str(ROOT / "tests" / "cpp" / "couplings_kat.cpp"), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for case in kat["cases"]:
+        args = [",".join(str(x) for x in case["charges"]), ",".join(str(x) for x in case["couplings"])]
+        out = subprocess.run([str(exe), *args], capture_output=True, text=True, check=True).stdout.splitlines()
+        rows = [ln.split() for ln in out]
+        assert int(next(r_[1] for r_ in rows if r_[0] == "how_many_couples")) == kat["how_many_couples"]
+        if "throws" in case:
+            assert ["throws", case["throws"]] in rows
+            continue
+        pattern = {(int(r_[1]), int(r_[2])): int(r_[3]) for r_ in rows if r_[0] == "couples"}
+        for m in range(kat["n_matter"]):
+            for g in range(kat["n_gauge"]):
+                assert pattern[(m, g)] == kat["pattern"][m * kat["n_gauge"] + g]
+        charge = {f"{r_[1]},{r_[2]}": float(r_[3]) for r_ in rows if r_[0] == "charge"}
+        for k, v in case["effective_charge"].items():
+            assert charge[k] == v, k
+        coupling = {r_[1]: float(r_[2]) for r_ in rows if r_[0] == "coupling"}
+        for k, v in case["coupling"].items():
+            assert coupling[k] == v, k
