@@ -201,3 +201,31 @@ def test_couplings_manager_matches_reference_kat(tmp_path):
         coupling = {r_[1]: float(r_[2]) for r_ in rows if r_[0] == "coupling"}
         for k, v in case["coupling"].items():
             assert coupling[k] == v, k
+
+
+def test_run_parameters_match_reference_parameter_lists(tmp_path):
+    """The mirror's RunParameters on the parameter lists of the reference's runparameters test
+    (tests/CosmoInterface/runparameters.cpp:23-37, lists in tests/golden/runparameters/): kIR = 1
+    gives lSide = 2 pi; no geometry, or kIR and lSide that disagree, are rejected; lSide = 2 gives
+    kIR = pi."""
+    import json
+
+    from cosmolattice_amd._build import HOST as host_dir
+
+    gold = ROOT / "tests" / "golden" / "runparameters"
+    expected = json.loads((gold / "expected.json").read_text())
+    exe = tmp_path / "runparams_probe"
+    r = subprocess.run(["g++", "-std=c++20", "-O1", "-Wall", f"-I{host_dir}", f"-I{ROOT / 'include'}",
+                        str(ROOT / "tests" / "cpp" / "runparams_probe.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for name, want in expected.items():
+        if name == "source":
+            continue
+        out = subprocess.run([str(exe), f"input={gold / name}"], capture_output=True, text=True, check=True).stdout
+        rows = dict(ln.split(None, 1) for ln in out.splitlines())
+        if "throws" in want:
+            assert rows.get("throws") == want["throws"], name
+        else:
+            for k, v in want.items():
+                assert float(rows[k]) == pytest.approx(v, rel=1e-15), (name, k)
