@@ -418,6 +418,26 @@ def main():
     acc["a_rel_diff"] = abs(bg32["aI"] - bg64["aI"]) / bg64["aI"]
     acc["t_end"] = (args.warmup + args.steps) * dt
 
+    # spectra (A14) and Gaussian-random-field initial conditions (A15) at 256^3: time per call
+    from cosmolattice_amd.clmi import PI_S
+    lat = lats["f64"]
+    lat.spectrum_bins(FLD_S, 1, 0, 1.0)  # plans and bin tables built on first use
+    lat.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        for k in (FLD_S, PI_S):
+            lat.spectrum_bins(k, 1, 0, 1.0)
+    spec_ms = (time.perf_counter() - t0) / 10 * 1e3
+    t0 = time.perf_counter()
+    tmp, _ = make_lattice(clmi, spec, N, kIR, dt, clmi.CLMI_F64, args.seed + 1)
+    tmp.synchronize()
+    init_s = time.perf_counter() - t0
+    tmp.close()
+    extras = {"spectrum_ms_per_field": spec_ms, "spectrum_modes": N * N * (N // 2 + 1),
+              "initial_conditions_s": init_s,
+              "note": "spectrum: R2C FFT + deterministic |k|-shell bin sums, host-synchronous call; initial "
+                      "conditions: lattice allocation, Philox GRF on both fields, two C2R FFTs each, Friedmann aDot"}
+
     r32, r64 = res["f32_bulk"], res["f64"]
     def roof(r, f32):
         avg_s = r["kernel_ms"] / 1e3 / max(r["launches"], 1)
@@ -470,6 +490,7 @@ def main():
         "f32_over_f64": r32["site_updates_per_s"] / r64["site_updates_per_s"],
         "hbm_frac_f32": roof(r32, True)["frac"],
         "accuracy": acc,
+        "spectra_and_ics": extras,
         "ensemble": ens,
         "u1": u1,
         "c1_vv2_64": c1,
