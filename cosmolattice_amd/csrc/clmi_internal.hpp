@@ -295,10 +295,17 @@ hipError_t launch_to_f32(const double *src, float *dst, uint32_t V, hipStream_t 
 hipError_t launch_join_f64(const float *hi, const float *lo, double *dst, uint32_t V, hipStream_t s);
 
 // Spectrum binning and Gaussian random fields (fft.hip).
+constexpr int kMaxStreamBins = 8192;  // streaming binning: one LDS histogram of <= 64 KiB per wave
 struct BinTable {
   int N = 0;
   double dk = 0;
   int nbins = 0;
+  // streaming pass (nbins <= kMaxStreamBins)
+  uint16_t *d_bin = nullptr;     // bin of each |n|^2, 0 <= |n|^2 <= 3 (N/2)^2
+  double *d_rows = nullptr;      // groups x nbins per-workgroup bin sums
+  int wpg = 0, groups = 0;       // waves per workgroup, workgroups
+  uint32_t chunks_per_wave = 0;  // 64-mode chunks per wave
+  // bin-sorted index table (more bins)
   uint32_t *d_modes = nullptr;   // half-space mode indices sorted by bin
   uint32_t *d_start = nullptr;   // nbins + 1 offsets
   std::vector<double> count, mean_n;

@@ -1,10 +1,16 @@
 // Kernels around the FFTs: |n|-shell binning of power spectra and Gaussian-random-field modes.
 //
-// Binning (measurements/powerspectrum.h:85-134 with TempLat's projectRadially): modes of the
-// half-complex R2C array are grouped by radial bin once per (N, dk) into a bin-sorted index table
-// (CSR).  A spectrum is then one block per bin summing w |f_k|^2 over its modes in a fixed order,
-// so bin sums are bit-reproducible (no floating-point atomics).  w = 1 on the kz = 0 and kz = N/2
-// planes and 2 elsewhere, so sums run over the full N^3 mode set.
+// Binning (measurements/powerspectrum.h:85-134 with TempLat's projectRadially): a spectrum is one
+// streaming pass over the half-complex R2C array, N*N lines of N/2+1 modes, z fastest.  A mode's bin
+// is looked up from |n|^2 in a table built on the host with the host's formula, so bins, counts and
+// mean |n| agree by construction.  Along a line |n| grows with kz, so each wave reduces a chunk of 64
+// consecutive modes with a segmented scan keyed by (line, bin) and adds the segment sums into its own
+// LDS histogram; the workgroup's histograms are added in wave order into one row per workgroup, and a
+// second kernel adds the rows in row order.  Every sum has a fixed order, so spectra are
+// bit-reproducible (no floating-point atomics).  w = 1 on the kz = 0 and kz = N/2 planes and 2
+// elsewhere, so sums run over the full N^3 mode set.  When the histograms do not fit in LDS (more than
+// kMaxStreamBins bins) the modes are grouped by bin once per (N, dk) into a bin-sorted index table
+// (CSR) and each bin is summed by one workgroup instead.
 //
 // Gaussian random field (initializers/fluctuationsgenerator.h:39-71,103-150): per mode two complex
 // Gaussian amplitudes fL, fR of rms sigma_k per real part; phi_k = (fL+fR)/sqrt2 and
@@ -25,56 +31,101 @@ hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins)
   free_bin_table(t);
   const uint32_t Nh = N / 2 + 1;
   const uint32_t M = (uint32_t)N * N * Nh;
-  std::vector<uint32_t> bin(M);
-  std::vector<uint32_t> cnt(nbins + 1, 0);
-  t.count.assign(nbins, 0.0);
-  t.mean_n.assign(nbins, 0.0);
-  std::vector<double> sumn(nbins, 0.0);
-  for (uint32_t kx = 0; kx < (uint32_t)N; ++kx) {
-    const int nx = kx <= (uint32_t)N / 2 ? (int)kx : (int)kx - N;
-    for (uint32_t ky = 0; ky < (uint32_t)N; ++ky) {
-      const int ny = ky <= (uint32_t)N / 2 ? (int)ky : (int)ky - N;
-      for (uint32_t kz = 0; kz < Nh; ++kz) {
-        const uint32_t m = (kx * N + ky) * Nh + kz;
-        const double nn = std::sqrt((double)(nx * nx + ny * ny + (int)(kz * kz)));
-        int b = (int)std::floor(nn / dk + 0.5);
-        if (b > nbins - 1) b = nbins - 1;
-        bin[m] = (uint32_t)b;
-        cnt[b + 1]++;
-        const double w = (kz == 0 || 2 * kz == (uint32_t)N) ? 1.0 : 2.0;
-        t.count[b] += w;
-        sumn[b] += w * nn;
-      }
+  const int half = N / 2;
+  const uint32_t n2max = 3u * (uint32_t)half * (uint32_t)half;
+  // bin of every |n|^2 on the lattice (the host formula, applied once per |n|^2)
+  std::vector<uint32_t> bin_of(n2max + 1);
+  for (uint32_t n2 = 0; n2 <= n2max; ++n2) {
+    const double nn = std::sqrt((double)n2);
+    int b = (int)std::floor(nn / dk + 0.5);
+    if (b > nbins - 1) b = nbins - 1;
+    bin_of[n2] = (uint32_t)b;
+  }
+  // weighted multiplicity of each |n|^2: the (nx, ny) pairs first, then the kz planes
+  std::vector<double> mult_r2(2u * (uint32_t)half * (uint32_t)half + 1, 0.0);
+  for (int kx = 0; kx < N; ++kx) {
+    const int nx = kx <= half ? kx : kx - N;
+    for (int ky = 0; ky < N; ++ky) {
+      const int ny = ky <= half ? ky : ky - N;
+      mult_r2[nx * nx + ny * ny] += 1.0;
     }
   }
-  for (int b = 0; b < nbins; ++b) {
-    t.mean_n[b] = t.count[b] > 0 ? sumn[b] / t.count[b] : 0.0;
-    cnt[b + 1] += cnt[b];
+  std::vector<double> mult(n2max + 1, 0.0);
+  for (uint32_t r2 = 0; r2 < mult_r2.size(); ++r2) {
+    if (mult_r2[r2] == 0.0) continue;
+    for (uint32_t kz = 0; kz < Nh; ++kz) {
+      const double w = (kz == 0 || 2 * kz == (uint32_t)N) ? 1.0 : 2.0;
+      mult[r2 + kz * kz] += w * mult_r2[r2];
+    }
   }
-  std::vector<uint32_t> start(cnt);  // offsets
+  t.count.assign(nbins, 0.0);
+  t.mean_n.assign(nbins, 0.0);
+  std::vector<long double> sumn(nbins, 0.0L);  // extended precision: mean |n| to about 1e-16
+  for (uint32_t n2 = 0; n2 <= n2max; ++n2) {
+    if (mult[n2] == 0.0) continue;
+    t.count[bin_of[n2]] += mult[n2];
+    sumn[bin_of[n2]] += (long double)mult[n2] * std::sqrt((long double)n2);
+  }
+  for (int b = 0; b < nbins; ++b) t.mean_n[b] = t.count[b] > 0 ? (double)(sumn[b] / t.count[b]) : 0.0;
+  t.N = N;
+  t.dk = dk;
+  t.nbins = nbins;
+  hipError_t e;
+  if (nbins <= kMaxStreamBins) {
+    // streaming pass: 4 waves per workgroup while four histograms fit in 64 KiB of LDS
+    t.wpg = (size_t)4 * nbins * sizeof(double) <= 65536 ? 4 : 1;
+    const uint64_t chunks = ((uint64_t)M + 63) / 64;
+    uint64_t waves = std::min<uint64_t>(4096, (chunks + 7) / 8);
+    uint64_t groups = (waves + t.wpg - 1) / t.wpg;
+    groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, (4u << 20) / (uint64_t)nbins));
+    t.groups = (int)std::max<uint64_t>(1, groups);
+    t.chunks_per_wave = (uint32_t)((chunks + (uint64_t)t.groups * t.wpg - 1) / ((uint64_t)t.groups * t.wpg));
+    const std::vector<uint16_t> bin16(bin_of.begin(), bin_of.end());
+    e = hipMalloc(&t.d_bin, sizeof(uint16_t) * bin16.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(t.d_bin, bin16.data(), sizeof(uint16_t) * bin16.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    return hipMalloc(&t.d_rows, sizeof(double) * (size_t)t.groups * nbins);
+  }
+  // bin-sorted index table
+  std::vector<uint32_t> start(nbins + 1, 0);
+  for (uint32_t kx = 0; kx < (uint32_t)N; ++kx) {
+    const int nx = kx <= (uint32_t)half ? (int)kx : (int)kx - N;
+    for (uint32_t ky = 0; ky < (uint32_t)N; ++ky) {
+      const int ny = ky <= (uint32_t)half ? (int)ky : (int)ky - N;
+      for (uint32_t kz = 0; kz < Nh; ++kz) start[bin_of[nx * nx + ny * ny + kz * kz] + 1]++;
+    }
+  }
+  for (int b = 0; b < nbins; ++b) start[b + 1] += start[b];
   std::vector<uint32_t> order(M);
-  std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
-  for (uint32_t m = 0; m < M; ++m) order[fill[bin[m]]++] = m;
-  hipError_t e = hipMalloc(&t.d_modes, sizeof(uint32_t) * (size_t)M);
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+  for (uint32_t kx = 0; kx < (uint32_t)N; ++kx) {
+    const int nx = kx <= (uint32_t)half ? (int)kx : (int)kx - N;
+    for (uint32_t ky = 0; ky < (uint32_t)N; ++ky) {
+      const int ny = ky <= (uint32_t)half ? (int)ky : (int)ky - N;
+      for (uint32_t kz = 0; kz < Nh; ++kz)
+        order[fill[bin_of[nx * nx + ny * ny + kz * kz]]++] = (kx * N + ky) * Nh + kz;
+    }
+  }
+  e = hipMalloc(&t.d_modes, sizeof(uint32_t) * (size_t)M);
   if (e != hipSuccess) return e;
   e = hipMalloc(&t.d_start, sizeof(uint32_t) * (size_t)(nbins + 1));
   if (e != hipSuccess) return e;
   e = hipMemcpy(t.d_modes, order.data(), sizeof(uint32_t) * (size_t)M, hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
-  e = hipMemcpy(t.d_start, start.data(), sizeof(uint32_t) * (size_t)(nbins + 1), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return e;
-  t.N = N;
-  t.dk = dk;
-  t.nbins = nbins;
-  return hipSuccess;
+  return hipMemcpy(t.d_start, start.data(), sizeof(uint32_t) * (size_t)(nbins + 1), hipMemcpyHostToDevice);
 }
 
 void free_bin_table(BinTable &t)
 {
   if (t.d_modes) (void)hipFree(t.d_modes);
   if (t.d_start) (void)hipFree(t.d_start);
+  if (t.d_bin) (void)hipFree(t.d_bin);
+  if (t.d_rows) (void)hipFree(t.d_rows);
   t.d_modes = nullptr;
   t.d_start = nullptr;
+  t.d_bin = nullptr;
+  t.d_rows = nullptr;
   t.N = 0;
   t.nbins = 0;
 }
@@ -102,6 +153,137 @@ __global__ __launch_bounds__(256) void k_bin_sums(const uint32_t *modes, const u
     __syncthreads();
   }
   if (threadIdx.x == 0) out[b] = sh[0];
+}
+
+// Lane-shifted copy of a double by a DPP row pattern (lanes outside the pattern or the row mask
+// read 0).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWMASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Streaming binning.  Wave w of workgroup g reduces the 64-mode chunks [c0, c0 + cpw) of the
+// half-space array (modes m = (kx N + ky) Nh + kz) into its LDS histogram h; the workgroup then writes
+// the sum of its WPG histograms, in wave order, to rows[g].
+template <int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_bin_stream(const double2 *__restrict__ fk,
+                                                         const uint16_t *__restrict__ bin_of, uint32_t N,
+                                                         uint32_t Nh, uint32_t M, uint32_t nbins, uint32_t cpw,
+                                                         double *__restrict__ rows)
+{
+  constexpr int U = 4;  // chunks loaded ahead
+  extern __shared__ double hist[];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *h = hist + wave * nbins;
+  for (uint32_t b = lane; b < nbins; b += 64) h[b] = 0.0;
+  const uint32_t half = N / 2;
+  const uint64_t nchunks = ((uint64_t)M + 63) / 64;
+  const uint64_t c0 = ((uint64_t)blockIdx.x * WPG + wave) * cpw;
+  const uint64_t cend = c0 + cpw < nchunks ? c0 + cpw : nchunks;
+  if (c0 < cend) {
+    // line (kx0, ky0) and kz0 of the current chunk's first mode
+    const uint64_t m0 = c0 * 64;
+    const uint32_t line = (uint32_t)(m0 / Nh);
+    uint32_t kz0 = (uint32_t)(m0 - (uint64_t)line * Nh);
+    uint32_t kx0 = line / N, ky0 = line - kx0 * N;
+    for (uint64_t c = c0; c < cend; c += U) {
+      double2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t m = (c + u) * 64 + lane;
+        v[u] = fk[(c + u < cend && m < M) ? m : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (c + u >= cend) break;
+        const bool valid = (c + u) * 64 + lane < M;
+        // this lane's line offset dl from the chunk's first line, and its kz
+        uint32_t kz = kz0 + lane, dl = 0;
+        while (kz >= Nh) {
+          kz -= Nh;
+          ++dl;
+        }
+        uint32_t ky = ky0 + dl, kx = kx0;
+        while (ky >= N) {
+          ky -= N;
+          ++kx;
+        }
+        const int nx = kx <= half ? (int)kx : (int)kx - (int)N;
+        const int ny = ky <= half ? (int)ky : (int)ky - (int)N;
+        const uint32_t n2 = valid ? (uint32_t)(nx * nx + ny * ny) + kz * kz : 0u;
+        const uint32_t b = bin_of[n2];
+        const double w = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
+        double x = valid ? w * (v[u].x * v[u].x + v[u].y * v[u].y) : 0.0;
+        // keys (line, bin) do not decrease along the chunk; invalid lanes form its tail
+        const uint32_t key = valid ? (dl << 16 | b) : 0xFFFFFFFFu;
+        const uint32_t kprev = __shfl_up(key, 1, 64);
+        const uint64_t heads = __ballot(lane == 0 || kprev != key);
+        const uint32_t start = 63 - __clzll((long long)(heads & ((2ull << lane) - 1)));
+        // segmented inclusive scan, x = sum over lanes [start, lane]: within rows of 16 lanes, then
+        // the last lane of rows 0 and 2 into rows 1 and 3, then lane 31 into rows 2 and 3
+        const uint32_t r = lane & 15;
+        double o = dpp_f64<0x111, 0xf>(x);  // row_shr:1
+        if (r >= 1 && lane >= start + 1) x += o;
+        o = dpp_f64<0x112, 0xf>(x);  // row_shr:2
+        if (r >= 2 && lane >= start + 2) x += o;
+        o = dpp_f64<0x114, 0xf>(x);  // row_shr:4
+        if (r >= 4 && lane >= start + 4) x += o;
+        o = dpp_f64<0x118, 0xf>(x);  // row_shr:8
+        if (r >= 8 && lane >= start + 8) x += o;
+        o = dpp_f64<0x142, 0xa>(x);  // row_bcast:15
+        if ((lane & 16) && (lane | 15) - 16 >= start) x += o;
+        o = dpp_f64<0x143, 0xc>(x);  // row_bcast:31
+        if (lane >= 32 && 31 >= start) x += o;
+        // a segment ends where the next lane starts one (invalid lanes start their own)
+        const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1));
+        // segments of one line have distinct bins; lines are added one after another
+        const uint32_t dlast = __builtin_amdgcn_readlane(dl, 63);
+        for (uint32_t j = 0; j <= dlast; ++j)
+          if (tail && dl == j) h[b] += x;
+        kz0 += 64;
+        while (kz0 >= Nh) {
+          kz0 -= Nh;
+          if (++ky0 == N) {
+            ky0 = 0;
+            ++kx0;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nbins; b += 64 * WPG) {
+    double s = hist[b];
+#pragma unroll
+    for (int w = 1; w < WPG; ++w) s += hist[w * nbins + b];
+    rows[(size_t)blockIdx.x * nbins + b] = s;
+  }
+}
+
+// out[b] = sum over rows g of rows[g][b] in a fixed order: wave q of 16 adds rows q, q + 16, ...,
+// then the 16 wave sums are added in wave order.
+__global__ __launch_bounds__(1024) void k_bin_rows(const double *__restrict__ rows, uint32_t groups, uint32_t nbins,
+                                                   double *__restrict__ out)
+{
+  __shared__ double sh[16][64];
+  const uint32_t l = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const uint32_t b = blockIdx.x * 64 + l;
+  double s = 0.0;
+  if (b < nbins) {
+#pragma unroll 8
+    for (uint32_t g = q; g < groups; g += 16) s += rows[(size_t)g * nbins + b];
+  }
+  sh[q][l] = s;
+  __syncthreads();
+  if (q == 0 && b < nbins) {
+    double t = sh[0][l];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) t += sh[k][l];
+    out[b] = t;
+  }
 }
 
 // ---- Philox4x32-10 ---------------------------------------------------------------------------
@@ -361,6 +543,19 @@ __global__ void k_grf_finish(const double *src, double scale, double add, void *
 
 hipError_t launch_bin_sums(const BinTable &t, const void *fk, double *out, hipStream_t s)
 {
+  if (t.d_bin) {
+    const uint32_t N = (uint32_t)t.N, Nh = N / 2 + 1, M = N * N * Nh;
+    const size_t lds = (size_t)t.wpg * t.nbins * sizeof(double);
+    if (t.wpg == 4)
+      hipLaunchKernelGGL(k_bin_stream<4>, dim3(t.groups), dim3(256), lds, s, (const double2 *)fk, t.d_bin, N, Nh, M,
+                         (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
+    else
+      hipLaunchKernelGGL(k_bin_stream<1>, dim3(t.groups), dim3(64), lds, s, (const double2 *)fk, t.d_bin, N, Nh, M,
+                         (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
+    hipLaunchKernelGGL(k_bin_rows, dim3((t.nbins + 63) / 64), dim3(1024), 0, s, t.d_rows, (uint32_t)t.groups,
+                       (uint32_t)t.nbins, out);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_bin_sums, dim3(t.nbins), dim3(256), 0, s, t.d_modes, t.d_start, (const double2 *)fk,
                      (uint32_t)(t.N / 2 + 1), (uint32_t)t.N, out);
   return hipGetLastError();
