@@ -39,6 +39,7 @@ def CLMI_FLAG_MEMMODE(m: int) -> int:
     return m << 8
 
 FLD_S, PI_S, FLD_CS, PI_CS, FLD_U1, PI_U1 = 0, 1, 2, 3, 6, 7
+MAG_U1 = 64  # derived kind: B = forward-difference curl of the U(1) field / dx
 
 # Reduced Planck mass in GeV, the reference's Constants::reducedMPlanck
 # (include/CosmoInterface/abstractmodel/scalefactorbase.h:18).
@@ -117,6 +118,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_gauss_u1": (C.c_int, [P, C.POINTER(C.c_double)]),
         "clmi_field_means": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
         "clmi_field_stats": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "clmi_norm_averages": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "clmi_monomial_averages": (C.c_int, [P, C.POINTER(C.c_double)]),
         "clmi_spectrum_nbins": (C.c_int, [P, C.c_double]),
         "clmi_spectrum_bins": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -408,6 +410,13 @@ class Lattice:
     def field_stats(self, kind: int, index: int, comp: int = 0) -> tuple:
         out = (C.c_double * 3)()
         self._check(self.lib.clmi_field_stats(self._h, kind, index, comp, out), "field_stats")
+        return tuple(out)
+
+    def norm_averages(self, kind: int, index: int = 0) -> tuple:
+        """(<|v|>, <|v|^2>) of the complex scalar (FLD_CS, PI_CS), the U(1) momenta (PI_U1) or the
+        magnetic field (MAG_U1)."""
+        out = (C.c_double * 2)()
+        self._check(self.lib.clmi_norm_averages(self._h, kind, index, out), "norm_averages")
         return tuple(out)
 
     def monomial_averages(self) -> list:

@@ -3,6 +3,8 @@
 // k_update finishes the per-block partial sums of the last lattice pass in a fixed order (so
 // results are bit-reproducible run to run), then one thread advances the scale factor exactly as
 // the reference's host code does, and refreshes the kick/drift coefficients the next pass reads.
+#include <algorithm>
+
 #include "clmi_internal.hpp"
 #include "update_device.hpp"
 
@@ -107,6 +109,63 @@ __global__ __launch_bounds__(kBlock) void k_means(const MeansArgs a)
   block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
 }
 
+// Component i of B = curl A at site s with forward differences: (d+_j A_k - d+_k A_j) / dx, (i, j, k)
+// cyclic.  ld(c, site) loads component c; nb[d] is the site one step forward in direction d.
+template <class LD>
+__device__ __forceinline__ double curl_component(const LD &ld, int i, uint32_t s, const uint32_t nb[3], double inv_dx)
+{
+  const int j = (i + 1) % 3, k = (i + 2) % 3;
+  return ((ld(k, nb[j]) - ld(k, s)) - (ld(j, nb[k]) - ld(j, s))) * inv_dx;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_norms(const NormArgs a)
+{
+  double acc[2] = {0, 0};
+  auto ld = [&](int c, uint32_t i) -> double {
+    if (F32) return (double)((const float *)a.p[c])[i] + (a.lo[c] ? (double)a.lo[c][i] : 0.0);
+    return ((const double *)a.p[c])[i];
+  };
+  const uint32_t N = a.N, NN = N * N;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
+    double q = 0.0;
+    if (!a.curl) {
+      for (int c = 0; c < a.ncomp; ++c) {
+        const double v = ld(c, s);
+        q += v * v;
+      }
+    } else {
+      const uint32_t x = s / NN, y = (s - x * NN) / N, z = s - x * NN - y * N;
+      const uint32_t nb[3] = {s - x * NN + (x + 1 == N ? 0 : x + 1) * NN, s - y * N + (y + 1 == N ? 0 : y + 1) * N,
+                              s - z + (z + 1 == N ? 0 : z + 1)};
+      for (int i = 0; i < 3; ++i) {
+        const double b = curl_component(ld, i, s, nb, a.inv_dx);
+        q += b * b;
+      }
+    }
+    acc[0] += sqrt(q);
+    acc[1] += q;
+  }
+  double v[kNumRed] = {acc[0], acc[1], 0, 0, 0, 0, 0, 0};
+  block_reduce_store<kNumRed>(v, a.partials + (size_t)blockIdx.x * kNumRed);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_curl_component(const NormArgs a, int comp, double *dst)
+{
+  auto ld = [&](int c, uint32_t i) -> double {
+    if (F32) return (double)((const float *)a.p[c])[i] + (a.lo[c] ? (double)a.lo[c][i] : 0.0);
+    return ((const double *)a.p[c])[i];
+  };
+  const uint32_t N = a.N, NN = N * N;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
+    const uint32_t x = s / NN, y = (s - x * NN) / N, z = s - x * NN - y * N;
+    const uint32_t nb[3] = {s - x * NN + (x + 1 == N ? 0 : x + 1) * NN, s - y * N + (y + 1 == N ? 0 : y + 1) * N,
+                            s - z + (z + 1 == N ? 0 : z + 1)};
+    dst[s] = curl_component(ld, comp, s, nb, a.inv_dx);
+  }
+}
+
 template <bool F32>
 __global__ __launch_bounds__(kBlock) void k_monomials(const MonoArgs a)
 {
@@ -180,6 +239,25 @@ hipError_t launch_monomials(const MonoArgs &a, int blocks, hipStream_t s)
     hipLaunchKernelGGL(k_monomials<true>, dim3(blocks), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_monomials<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_norms(const NormArgs &a, int blocks, hipStream_t s)
+{
+  if (a.f32)
+    hipLaunchKernelGGL(k_norms<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_norms<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_curl_component(const NormArgs &a, int comp, double *dst, hipStream_t s)
+{
+  const uint32_t blocks = std::min<uint32_t>((a.V + kBlock - 1) / kBlock, 16384u);
+  if (a.f32)
+    hipLaunchKernelGGL(k_curl_component<true>, dim3(blocks), dim3(kBlock), 0, s, a, comp, dst);
+  else
+    hipLaunchKernelGGL(k_curl_component<false>, dim3(blocks), dim3(kBlock), 0, s, a, comp, dst);
   return hipGetLastError();
 }
 

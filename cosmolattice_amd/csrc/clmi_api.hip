@@ -900,6 +900,62 @@ int clmi_field_stats(clmi_ctx *c, int kind, int index, int comp, double out[3])
   return CLMI_OK;
 }
 
+namespace {
+// The three U(1) field components as a curl input (FLD_U1, current buffers).
+int u1_curl_args(clmi_ctx *c, int index, NormArgs *a)
+{
+  for (int d = 0; d < 3; ++d) {
+    DevArr *arr = nullptr;
+    bool is_field = false;
+    CHK(select_arr(c, CLMI_FLD_U1, index, d, &arr, &is_field));
+    a->p[d] = arr->p;
+    a->lo[d] = arr->lo;
+  }
+  a->ncomp = 3;
+  a->curl = 1;
+  a->f32 = c->f32;
+  a->N = (uint32_t)c->N;
+  a->V = c->V;
+  a->inv_dx = 1.0 / c->cfg.dx;
+  return CLMI_OK;
+}
+}  // namespace
+
+int clmi_norm_averages(clmi_ctx *c, int kind, int index, double out[2])
+{
+  CHK(check_ctx(c));
+  if (!out) return fail(c, CLMI_E_INVALID, "null output");
+  CHK(flush(c));
+  NormArgs a{};
+  if (kind == CLMI_MAG_U1) {
+    CHK(u1_curl_args(c, index, &a));
+  } else if (kind == CLMI_FLD_CS || kind == CLMI_PI_CS || kind == CLMI_PI_U1) {
+    a.ncomp = kind == CLMI_PI_U1 ? 3 : 2;
+    for (int d = 0; d < a.ncomp; ++d) {
+      DevArr *arr = nullptr;
+      bool is_field = false;
+      CHK(select_arr(c, kind, index, d, &arr, &is_field));
+      a.p[d] = arr->p;
+      a.lo[d] = arr->lo;
+    }
+    a.f32 = c->f32;
+    a.N = (uint32_t)c->N;
+    a.V = c->V;
+  } else {
+    return fail(c, CLMI_E_INVALID, "clmi_norm_averages: kind must be CLMI_FLD_CS, CLMI_PI_CS, CLMI_PI_U1 or CLMI_MAG_U1");
+  }
+  a.partials = c->partials;
+  const int blocks = scalar_pass_blocks(c->V);
+  HIPCHK(c, launch_norms(a, blocks, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, blocks, c->dscratch, c->stream));
+  double h[kNumRed];
+  HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = h[0] / (double)c->V;
+  out[1] = h[1] / (double)c->V;
+  return CLMI_OK;
+}
+
 int clmi_monomial_averages(clmi_ctx *c, double *out)
 {
   CHK(check_ctx(c));
@@ -939,14 +995,22 @@ int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, in
   if (!(dk > 0) || nbins != clmi_spectrum_nbins(c, dk) || !sum_f2)
     return fail(c, CLMI_E_INVALID, "nbins must equal clmi_spectrum_nbins(ctx, dk)");
   CHK(flush(c));
-  DevArr *arr = nullptr;
-  bool is_field = false;
-  CHK(select_arr(c, kind, index, comp, &arr, &is_field));
-  CHK(ensure_fft(c, true, false, false));
-  if (c->f32)
-    HIPCHK(c, launch_join_f64((const float *)arr->p, arr->lo, c->stage, c->V, c->stream));
-  else
-    HIPCHK(c, hipMemcpyAsync(c->stage, arr->p, sizeof(double) * c->V, hipMemcpyDeviceToDevice, c->stream));
+  if (kind == CLMI_MAG_U1) {
+    if (comp < 0 || comp > 2) return fail(c, CLMI_E_INVALID, "CLMI_MAG_U1: comp must be 0, 1 or 2");
+    NormArgs a{};
+    CHK(u1_curl_args(c, index, &a));
+    CHK(ensure_fft(c, true, false, false));
+    HIPCHK(c, launch_curl_component(a, comp, c->stage, c->stream));
+  } else {
+    DevArr *arr = nullptr;
+    bool is_field = false;
+    CHK(select_arr(c, kind, index, comp, &arr, &is_field));
+    CHK(ensure_fft(c, true, false, false));
+    if (c->f32)
+      HIPCHK(c, launch_join_f64((const float *)arr->p, arr->lo, c->stage, c->V, c->stream));
+    else
+      HIPCHK(c, hipMemcpyAsync(c->stage, arr->p, sizeof(double) * c->V, hipMemcpyDeviceToDevice, c->stream));
+  }
   FFTCHK(c, hipfftExecD2Z(c->plan_r2c, (hipfftDoubleReal *)c->stage, (hipfftDoubleComplex *)c->fk));
   if (c->bins.nbins != nbins || c->bins.dk != dk || c->bins.N != c->N) {
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -287,6 +287,19 @@ struct MonoArgs {  // slot t = sum over sites of monomial t of the scalar potent
 };
 hipError_t launch_monomials(const MonoArgs &a, int blocks, hipStream_t s);
 hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s);
+// Norm statistics of a 2- or 3-component field, or of the forward-difference curl of a 3-component
+// field (curl = 1): slot 0 = sum over sites of |v|, slot 1 = sum of |v|^2.
+struct NormArgs {
+  const void *p[3];
+  const float *lo[3];  // low words of hi/lo arrays (FP32-bulk fields), else null
+  int ncomp, curl, f32;
+  uint32_t N, V;
+  double inv_dx;
+  double *partials;
+};
+hipError_t launch_norms(const NormArgs &a, int blocks, hipStream_t s);
+// dst[s] = component comp of the forward-difference curl of (p[0], p[1], p[2]) divided by dx.
+hipError_t launch_curl_component(const NormArgs &a, int comp, double *dst, hipStream_t s);
 hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s);
 
 // Array conversion kernels (upload / download in F32-bulk mode, FFT staging).

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <fstream>
 #include <iomanip>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -107,10 +108,25 @@ public:
   {
     if (outputFreq < 1 || infreqOutputFreq < 1 || rareOutputFreq < 1)
       throw RunParametersInconsistent("output intervals must be at least one time step");
+    const std::vector<std::string> header{"t", "<phi>", "<pi>", "<phi^2>", "<pi^2>", "rms(phi)", "rms(pi)"};
     for (size_t i = 0; i < Model::Ns; ++i)
-      means.emplace_back(par.outFn + "average_scalar_" + std::to_string(i) + ".txt",
-                         std::vector<std::string>{"t", "<phi>", "<pi>", "<phi^2>", "<pi^2>", "rms(phi)", "rms(pi)"},
+      means.emplace_back(par.outFn + "average_scalar_" + std::to_string(i) + ".txt", header, par.printHeaders);
+    // complexscalarmeasurer.h:35-56: real part, imaginary part and norm of each complex scalar
+    for (size_t i = 0; i < Model::NCs; ++i) {
+      const std::string n = "cmplx_scalar_" + std::to_string(i);
+      csMeans.emplace_back(par.outFn + "average_" + n + "_0.txt", header, par.printHeaders);
+      csMeans.emplace_back(par.outFn + "average_" + n + "_1.txt", header, par.printHeaders);
+      csMeans.emplace_back(par.outFn + "average_norm_" + n + ".txt", header, par.printHeaders);
+    }
+    // u1measurer.h:42-62: norms of E and B, and the Gauss law, of each U(1) field
+    for (size_t i = 0; i < Model::NU1; ++i) {
+      u1Means.emplace_back(par.outFn + "average_norm_U1_" + std::to_string(i) + ".txt",
+                           std::vector<std::string>{"t", "<|E|>", "<|B|>", "<|E|^2>", "<|B|^2>", "rms(|E|)", "rms(|B|)"},
+                           par.printHeaders);
+      gauss.emplace_back(par.outFn + "average_gauss_U1_" + std::to_string(i) + ".txt",
+                         std::vector<std::string>{"t", "av|LHS-RHS|_over_av|LHS+RHS|", "av|LHS|", "av|RHS|"},
                          par.printHeaders);
+    }
   }
 
   // measurer.h:181-197
@@ -148,6 +164,46 @@ private:
     return h;
   }
 
+  // MeansMeasurer::measure (meansmeasurer.h:28-48) from <f>, <f^2>, <p>, <p^2>:
+  // t, <f>, <p>, <f^2>, <p^2>, rms(f), rms(p), with a negative variance written as rms 0.
+  static std::vector<double> meansRow(double t, double f1, double f2, double p1, double p2)
+  {
+    const double rf = f2 - f1 * f1, rp = p2 - p1 * p1;
+    return {t, f1, p1, f2, p2, rf > 0 ? std::sqrt(rf) : 0.0, rp > 0 ? std::sqrt(rp) : 0.0};
+  }
+
+  // Complex scalars (complexscalarmeasurer.h:71-79): the components sqrt2 phi_c and
+  // sqrt2 a^(alpha-3) pi_c, then |phi| and a^(alpha-3) |pi|.  U(1) fields (u1measurer.h:70-86):
+  // |E| = a^(alpha-1) |pi_A| and |B|, then the Gauss law.
+  void measureGaugeSector(T t, Model &model)
+  {
+    clmi_ctx *ctx = model.context();
+    const double a = model.aI, al = model.alpha;
+    for (size_t i = 0; i < Model::NCs; ++i) {
+      const double sp = std::sqrt(2.0) * std::pow(a, al - 3);
+      for (int c = 0; c < 2; ++c) {
+        double f[2], p[2];
+        model.check(clmi_field_means(ctx, CLMI_FLD_CS, (int)i, c, std::sqrt(2.0), f), "clmi_field_means");
+        model.check(clmi_field_means(ctx, CLMI_PI_CS, (int)i, c, sp, p), "clmi_field_means");
+        csMeans[3 * i + c].save(meansRow(t, f[0], f[1], p[0], p[1]));
+      }
+      double f[2], p[2];
+      model.check(clmi_norm_averages(ctx, CLMI_FLD_CS, (int)i, f), "clmi_norm_averages");
+      model.check(clmi_norm_averages(ctx, CLMI_PI_CS, (int)i, p), "clmi_norm_averages");
+      const double s = std::pow(a, al - 3);
+      csMeans[3 * i + 2].save(meansRow(t, f[0], f[1], s * p[0], s * s * p[1]));
+    }
+    for (size_t i = 0; i < Model::NU1; ++i) {
+      double e[2], b[2], g[3];
+      model.check(clmi_norm_averages(ctx, CLMI_PI_U1, (int)i, e), "clmi_norm_averages");
+      model.check(clmi_norm_averages(ctx, CLMI_MAG_U1, (int)i, b), "clmi_norm_averages");
+      const double s = std::pow(a, al - 1);
+      u1Means[i].save(meansRow(t, s * e[0], s * s * e[1], b[0], b[1]));
+      model.check(clmi_gauss_u1(ctx, g), "clmi_gauss_u1");
+      gauss[i].save({(double)t, g[0], g[1], g[2]});
+    }
+  }
+
   // Frequent output: scalar means (meansmeasurer.h:28-48, scalarsingletmeasurer.h:60-66), energies
   // (energiesmeasurer.h:45-103), energy conservation (energiesmeasurer.h:105-122), scale factor.
   void measureStandard(T t, Model &model)
@@ -183,6 +239,7 @@ private:
       row.push_back(e.mag);
       etot += e.elec + e.mag;
     }
+    measureGaugeSector(t, model);
     std::vector<double> mono(CLMI_MAX_TERMS, 0.0);
     if (Model::Ns > 0) model.check(clmi_monomial_averages(ctx, mono.data()), "clmi_monomial_averages");
     const auto &tm = model.termMonomials();
@@ -212,25 +269,63 @@ private:
     }
   }
 
+  // One block of a spectra file (spectrumsaverstd.h:54-96): a row per bin with k and the columns,
+  // then an empty line.  The header line is written once, before the first block.
+  void saveSpectra(const std::string &name, T t, const std::vector<double> &k,
+                   const std::vector<std::vector<double>> &cols, const std::vector<std::string> &names)
+  {
+    const std::string fn = dir + "spectra_" + name + ".txt";
+    const bool started = spectraStarted_.count(fn) > 0;
+    std::ofstream out(fn, started ? std::ios_base::app : std::ios_base::out);
+    if (!out) throw std::runtime_error("cannot open " + fn);
+    out << std::setprecision(15);
+    if (!started && printHeaders) {
+      out << "#k";
+      for (const auto &n : names) out << "    " << n;
+      out << "    t = " << t << "\n";
+    }
+    for (size_t b = 0; b < k.size(); ++b) {
+      out << k[b];
+      for (const auto &c : cols) out << "    " << c[b];
+      out << "\n";
+    }
+    out << "\n";
+    spectraStarted_.insert(fn);
+  }
+
   // Infrequent output: spectra of every scalar and its momentum (scalarsingletmeasurer.h:74-83),
-  // k, PS(phi), a^(2 alpha - 6) PS(pi) per bin, one block per time.
+  // k, PS(phi), a^(2 alpha - 6) PS(pi) per bin; of the complex scalar's norm
+  // (complexscalarmeasurer.h:85-95), summed over its two components; of the U(1) electric and magnetic
+  // fields (u1measurer.h:91-105), a^(2 alpha - 2) sum_i PS(pi_A,i) and sum_i PS(B_i).
   void measureSpectra(T t, Model &model)
   {
     const double a = model.aI, al = model.alpha;
+    std::vector<double> k, pf, pp;
+    auto sum = [&](int kind, int idx, int ncomp, double scale) {
+      std::vector<double> acc, c;
+      for (int q = 0; q < ncomp; ++q) {
+        powerSpectrum(model, kind, idx, q, dk, psType, psVersion, k, c);
+        if (acc.empty()) acc.assign(c.size(), 0.0);
+        for (size_t b = 0; b < c.size(); ++b) acc[b] += c[b];
+      }
+      for (auto &v : acc) v *= scale;
+      return acc;
+    };
     for (size_t i = 0; i < Model::Ns; ++i) {
-      std::vector<double> k, pf, pp;
-      powerSpectrum(model, CLMI_FLD_S, (int)i, 0, dk, psType, psVersion, k, pf);
-      powerSpectrum(model, CLMI_PI_S, (int)i, 0, dk, psType, psVersion, k, pp);
-      const std::string fn = dir + "spectra_scalar_" + std::to_string(i) + ".txt";
-      std::ofstream out(fn, spectraStarted_ ? std::ios_base::app : std::ios_base::out);
-      if (!out) throw std::runtime_error("cannot open " + fn);
-      out << std::setprecision(15);
-      if (!spectraStarted_ && printHeaders) out << "#k    PS(phi)    PS(pi)    t = " << t << "\n";
-      for (size_t b = 0; b < k.size(); ++b)
-        out << k[b] << "    " << pf[b] << "    " << std::pow(a, 2 * al - 6) * pp[b] << "\n";
-      out << "\n";
+      pf = sum(CLMI_FLD_S, (int)i, 1, 1.0);
+      pp = sum(CLMI_PI_S, (int)i, 1, std::pow(a, 2 * al - 6));
+      saveSpectra("scalar_" + std::to_string(i), t, k, {pf, pp}, {"PS(phi)", "PS(pi)"});
     }
-    spectraStarted_ = true;
+    for (size_t i = 0; i < Model::NCs; ++i) {
+      pf = sum(CLMI_FLD_CS, (int)i, 2, 1.0);
+      pp = sum(CLMI_PI_CS, (int)i, 2, std::pow(a, 2 * al - 6));
+      saveSpectra("norm_cmplx_scalar_" + std::to_string(i), t, k, {pf, pp}, {"PS(|phi|)", "PS(|pi|)"});
+    }
+    for (size_t i = 0; i < Model::NU1; ++i) {
+      const auto pe = sum(CLMI_PI_U1, (int)i, 3, std::pow(a, 2 * al - 2));
+      const auto pb = sum(CLMI_MAG_U1, (int)i, 3, 1.0);
+      saveSpectra("norm_U1_" + std::to_string(i), t, k, {pe, pb}, {"PS(E)", "PS(B)"});
+    }
   }
 
   bool expansion, printHeaders;
@@ -238,8 +333,9 @@ private:
   int psType, psVersion;
   std::string dir;
   AverageFile energyCons, scaleFactor, energies;
-  std::vector<AverageFile> means;
-  bool first_ = true, spectraStarted_ = false;
+  std::vector<AverageFile> means, csMeans, u1Means, gauss;
+  std::set<std::string> spectraStarted_;
+  bool first_ = true;
   double etot0_ = 0.0;
 };
 

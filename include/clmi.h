@@ -65,6 +65,12 @@ extern "C" {
 #define CLMI_PI_CS 3
 #define CLMI_FLD_U1 6
 #define CLMI_PI_U1 7
+/* Derived kind (not a FieldsNumbering value) for clmi_norm_averages and clmi_spectrum_bins: the
+ * magnetic field of the U(1) field, B_i = (d+_j A_k - d+_k A_j)/dx with (i, j, k) cyclic and forward
+ * differences d+_j f = f(x+j) - f(x), so |B|^2 = F_12^2 + F_13^2 + F_23^2, the field strength of
+ * definitions/fieldfunctionals.h:140-153 (TempLat's magneticField as measurements/u1measurer.h:72,95-97
+ * uses it).  comp selects B_1, B_2, B_3. */
+#define CLMI_MAG_U1 64
 
 /* Model and run configuration.  Replaces the model/run state the reference builds in
  * AbstractModel (abstractmodel.h:147-186), RunParameters (runparameters.h:60-236) and the
@@ -190,6 +196,14 @@ int clmi_field_means(clmi_ctx *ctx, int kind, int index, int comp, double scale,
  * inputs of MeansMeasurer (measurements/meansmeasurer.h:28-48) and of the per-field gradient energy
  * (definitions/fieldfunctionals.h:52-67, energies.h:37-45). */
 int clmi_field_stats(clmi_ctx *ctx, int kind, int index, int comp, double out[3]);
+/* Norm statistics of a vector-valued field, the inputs MeansMeasurer::measure takes for norm(...)
+ * (measurements/meansmeasurer.h:28-48): out = {<|v|>, <|v|^2>}.  kind CLMI_FLD_CS / CLMI_PI_CS:
+ * |phi| = sqrt(phi_0^2 + phi_1^2) of complex scalar `index` and the same for its momentum
+ * (measurements/complexscalarmeasurer.h:71-79).  kind CLMI_PI_U1: |pi_A| of U(1) field `index`;
+ * kind CLMI_MAG_U1: |B| (measurements/u1measurer.h:70-73).  Scale factors (a^(alpha-3),
+ * a^(alpha-1)) are the caller's. */
+int clmi_norm_averages(clmi_ctx *ctx, int kind, int index, double out[2]);
+
 /* Volume average of every scalar-potential monomial prod_f phi_f^term_exp[t][f] (without its
  * coefficient), t = 0..n_terms-1: the potential terms of EnergiesMeasurer (energiesmeasurer.h:85-90)
  * are sums of these with the model's coefficients. */
@@ -200,7 +214,8 @@ int clmi_monomial_averages(clmi_ctx *ctx, double *out);
 /* Number of radial bins used by clmi_spectrum_bins for bin width dk: floor(sqrt(3)/2*N/dk) + 1
  * (runparameters.h:97-98 gives the reference's floor(sqrt(3)/2*N/dk); one more bin holds k = 0). */
 int clmi_spectrum_nbins(const clmi_ctx *ctx, double dk);
-/* R2C FFT of one component (unnormalised forward transform) and deterministic |n|-shell binning
+/* R2C FFT of one component (unnormalised forward transform; kind CLMI_MAG_U1 transforms B_comp) and
+ * deterministic |n|-shell binning
  * (measurements/powerspectrum.h:85-134 with TempLat's projectRadially).  Mode n (integer vector,
  * components in (-N/2, N/2]) goes to bin b = min(floor(|n|/dk + 1/2), nbins-1).  Outputs, each of
  * length nbins: sum over the full N^3 mode set of |f_n|^2, number of modes, and the mean |n| of

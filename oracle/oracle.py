@@ -274,6 +274,15 @@ def mode_norms(N: int) -> np.ndarray:
     return np.sqrt(nx * nx + ny * ny + nz * nz)
 
 
+def magnetic_field(A, dx: float):
+    """B_i = (d+_j A_k - d+_k A_j)/dx with (i, j, k) cyclic and forward differences, so that
+    |B|^2 = F_12^2 + F_13^2 + F_23^2 with the field strength F_ij of definitions/fieldfunctionals.h:140-153
+    (TempLat's magneticField as measurements/u1measurer.h:72,95-97 uses it)."""
+    def fd(f, ax):
+        return np.roll(f, -1, axis=ax) - f
+    return [(fd(A[(i + 2) % 3], (i + 1) % 3) - fd(A[(i + 1) % 3], (i + 2) % 3)) / dx for i in range(3)]
+
+
 def spectrum_bins(f: np.ndarray, dk: float = 1.0):
     """|f_n|^2 summed over |n|-shells of the unnormalised forward DFT (full N^3 mode set); returns
     (sum |f_n|^2, number of modes, mean |n|) per bin, bin b = min(floor(|n|/dk + 1/2), nbins-1)."""

@@ -231,6 +231,51 @@ def test_host_lphi4u1_initializer(gpu, oracle_mod, tmp_path):
     assert orc.gauss_u1()[0] < 1e-13
 
 
+def test_host_lphi4u1_measurement_files(gpu, tmp_path):
+    """The lphi4U1 program writes the complex-scalar and U(1) files of complexscalarmeasurer.h and
+    u1measurer.h at the measurement cadence: component means, norms, Gauss law and norm spectra.  Norms
+    and component means come from separate reductions and must agree; <|B|^2> must agree with the
+    magnetic energy of average_energies.txt; the Gauss law holds for the program's initial conditions."""
+    say("case host lphi4U1 measurement files")
+    from cosmolattice_amd.clmi import lphi4U1
+
+    N, kIR, dt = 32, 0.4, 0.005
+    phi0 = 5.6964e18
+    g = math.sqrt(10.0)
+    params = (f"N = {N}\nkIR = {kIR!r}\ndt = {dt!r}\ntMax = {4 * dt!r}\ntOutputFreq = {2 * dt!r}\n"
+              f"tOutputInfreq = {2 * dt!r}\ntOutputRareFreq = {2 * dt!r}\nexpansion = false\nlambda = 1\n"
+              f"gU1s = {g!r}\nCSU1_charges = 1\ncmplx_field_initial_norm = {phi0!r}\n"
+              f"cmplx_momentum_initial_norm = 0\nbaseSeed = 3\nkCutOff = {N * kIR / 4!r}\nprint_headers = true\n")
+    run_program("lphi4U1", tmp_path, params)
+    spec = lphi4U1(10.0, phi0_gev=phi0, lam=1.0, pi0_gev2=0.0)
+    w2 = (spec.omegaStar / spec.fStar) ** 2
+    load = lambda n: np.atleast_2d(np.loadtxt(tmp_path / n))
+    re, im = load("average_cmplx_scalar_0_0.txt"), load("average_cmplx_scalar_0_1.txt")
+    nrm = load("average_norm_cmplx_scalar_0.txt")
+    u1 = load("average_norm_U1_0.txt")
+    gs = load("average_gauss_U1_0.txt")
+    en = load("average_energies.txt")
+    for a in (re, im, nrm, u1, gs, en):
+        assert a.shape[0] == 2 and np.allclose(a[:, 0], [0.0, 2 * dt])
+    # components carry sqrt2: <2 phi_0^2> + <2 phi_1^2> = 2 <|phi|^2>, the same for the momenta
+    np.testing.assert_allclose(re[:, 3] + im[:, 3], 2 * nrm[:, 3], rtol=1e-12)
+    np.testing.assert_allclose(re[:, 4] + im[:, 4], 2 * nrm[:, 4], rtol=1e-12)
+    assert np.all(nrm[:, 1] <= np.sqrt(nrm[:, 3]) * (1 + 1e-14))  # <|phi|> <= sqrt(<|phi|^2>)
+    # E_B = w^2 <|B|^2> / 2 (a = 1); A starts at 0 and grows
+    assert u1[0, 4] == 0.0 and u1[1, 4] > 0.0
+    header = (tmp_path / "average_energies.txt").read_text().splitlines()[0].lstrip("#").split()
+    np.testing.assert_allclose(0.5 * w2 * u1[:, 4], en[:, header.index("E^grad_U10")], rtol=1e-12)
+    np.testing.assert_allclose(0.5 * w2 * u1[:, 3], en[:, header.index("E^kin_U10")], rtol=1e-12)
+    assert np.all(gs[:, 1] < 1e-10)
+    for name in ("spectra_norm_cmplx_scalar_0.txt", "spectra_norm_U1_0.txt"):
+        text = (tmp_path / name).read_text()
+        blocks = [b for b in text.split("\n\n") if b.strip()]
+        assert len(blocks) == 2
+        rows = np.loadtxt((tmp_path / name).open(), comments="#")
+        assert rows.shape == (2 * (int(math.floor(math.sqrt(3) / 2 * N)) + 1), 3)
+        assert np.all(np.isfinite(rows)) and rows[:, 1].max() > 0
+
+
 def test_host_backup_and_restart(gpu, tmp_path):
     """A run restarted from its own backup (load_file, as SimulationManager::loadSim) ends where the
     uninterrupted run ends, to rounding: the restart recomputes the cached averages

@@ -9,7 +9,7 @@ import math
 import numpy as np
 import pytest
 
-from cosmolattice_amd.clmi import (CLMI_F32_BULK, CLMI_F64, FLD_CS, FLD_S, FLD_U1, PI_CS, PI_S, PI_U1,
+from cosmolattice_amd.clmi import (CLMI_F32_BULK, CLMI_F64, FLD_CS, FLD_S, FLD_U1, MAG_U1, PI_CS, PI_S, PI_U1,
                                    Lattice, ModelSpec, lphi4, lphi4U1)
 
 pytestmark = pytest.mark.gpu
@@ -543,6 +543,43 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
             np.testing.assert_allclose(m, mo, rtol=1e-12)
             np.testing.assert_allclose(s, so, rtol=1e-12, atol=1e-12 * so.max())
             lat.close()
+
+
+@pytest.mark.parametrize("prec", [CLMI_F64, CLMI_F32_BULK])
+def test_norm_averages_and_magnetic_spectrum(gpu, oracle_mod, prec):
+    """<|v|>, <|v|^2> of the complex scalar, its momentum, the U(1) momenta and the magnetic field
+    equal numpy on the lattice's own arrays; <|B|^2> equals the evolver's field-strength reduction;
+    the spectrum of each B component equals the oracle's binning of numpy's B."""
+    say("case norms", prec)
+    spec = u1_spec()
+    N, dx = 32, 0.7
+    rng = np.random.default_rng(17)
+    st = u1_ics(spec, N, rng)
+    for d in range(3):
+        st[(FLD_U1, d)] = st[(FLD_U1, d)] + 0.02 * rng.normal(size=(N, N, N))
+    lat = Lattice(spec, N, dx, 0.01, prec)
+    load_u1(lat, st)
+    got = {(k, c): lat.download(k, 0, c) for (k, c) in st}
+    for kind, nc in ((FLD_CS, 2), (PI_CS, 2), (PI_U1, 3)):
+        q = sum(got[(kind, c)] ** 2 for c in range(nc))
+        m1, m2 = lat.norm_averages(kind)
+        assert m1 == pytest.approx(np.mean(np.sqrt(q)), rel=1e-12)
+        assert m2 == pytest.approx(np.mean(q), rel=1e-12)
+    B = oracle_mod.magnetic_field([got[(FLD_U1, d)] for d in range(3)], dx)
+    q = B[0] ** 2 + B[1] ** 2 + B[2] ** 2
+    m1, m2 = lat.norm_averages(MAG_U1)
+    assert m1 == pytest.approx(np.mean(np.sqrt(q)), rel=1e-12)
+    assert m2 == pytest.approx(np.mean(q), rel=1e-12)
+    # the evolver's FP32-bulk reduction rounds each field-strength difference to FP32
+    assert m2 == pytest.approx(lat.averages()["B2U1"], rel=1e-12 if prec == CLMI_F64 else 1e-6)
+    for c in range(3):
+        s, cnt, _ = lat.spectrum_bins(MAG_U1, 0, c, 1.0)
+        so, co, _ = oracle_mod.spectrum_bins(B[c], 1.0)
+        np.testing.assert_array_equal(cnt, co)
+        np.testing.assert_allclose(s, so, rtol=1e-11, atol=1e-12 * so.max())
+    with pytest.raises(Exception):
+        lat.norm_averages(FLD_S)
+    lat.close()
 
 
 def test_grf_parity(gpu, oracle_mod):

@@ -419,6 +419,31 @@ def test_u1_gauge_invariance(oracle_mod):
         np.testing.assert_allclose(b.download(PI_U1, 0, d), a.download(PI_U1, 0, d), atol=1e-11)
 
 
+def test_magnetic_field_pinned(oracle_mod):
+    """The magnetic field of the measurements: |B|^2 averages to the C oracle's field-strength
+    reduction B2U1, it is gauge invariant, and a plane wave A_z = sin(2 pi n x/N) has the closed-form
+    B_y = -(sin(2 pi n (x+1)/N) - sin(2 pi n x/N))/dx with B_x = B_z = 0."""
+    N, dx = 8, 0.7
+    spec = _u1_spec()
+    rng = np.random.default_rng(5)
+    st = _u1_state(N, rng)
+    A = [st[(FLD_U1, d)] for d in range(3)]
+    B = oracle_mod.magnetic_field(A, dx)
+    a = oracle_mod.OracleLattice(spec, N, dx, 0.01, expansion=False)
+    _load(a, st)
+    assert np.mean(B[0] ** 2 + B[1] ** 2 + B[2] ** 2) == pytest.approx(a.averages()["B2U1"], rel=1e-13)
+    chi = rng.uniform(-3, 3, size=(N, N, N))
+    Bg = oracle_mod.magnetic_field([A[d] + (np.roll(chi, -1, axis=d) - chi) for d in range(3)], dx)
+    for d in range(3):
+        np.testing.assert_allclose(Bg[d], B[d], atol=1e-12)
+    n = 3
+    x = np.arange(N)[:, None, None] * np.ones((1, N, N))
+    Az = np.sin(2 * np.pi * n * x / N)
+    Bp = oracle_mod.magnetic_field([np.zeros_like(Az), np.zeros_like(Az), Az], dx)
+    np.testing.assert_allclose(Bp[1], -(np.sin(2 * np.pi * n * (x + 1) / N) - Az) / dx, atol=1e-13)
+    assert np.abs(Bp[0]).max() == 0 and np.abs(Bp[2]).max() == 0
+
+
 def test_u1_gauss_law_conserved(oracle_mod):
     """Initial data obeying the discrete Gauss law (pi_A = 0, charge density zero) keep obeying it
     under leapfrog evolution: the kick's current and the charge density satisfy the discrete
