@@ -11,21 +11,24 @@
 namespace clmi {
 namespace {
 
-constexpr int kUpdThreads = 1024;
+constexpr int kUpdThreads = 256;
 
 __global__ __launch_bounds__(kUpdThreads) void k_update(DevState *st, const double *partials, int rows,
                                                         ModelConsts mc, UpdateOps ops)
 {
   __shared__ double tot[kNumRed];
   const int t = threadIdx.x;
-  if (rows > 0) {
-    sum_partials<kUpdThreads>(partials, rows, tot);
-    if (t < kNumRed) st->red[t] = tot[t] * mc.invV;
-  }
+  // Thread 0 reads the state before the partial sums, so that its loads overlap theirs, and writes it
+  // back once at the end.
+  DevState s;
+  if (t == 0) s = *st;
+  if (rows > 0) sum_partials<kUpdThreads>(partials, rows, tot);
   if (t != 0) return;
   double red[kNumRed];
-  for (int k = 0; k < kNumRed; ++k) red[k] = rows > 0 ? tot[k] * mc.invV : st->red[k];
-  apply_update(st, red, mc, ops);
+  for (int k = 0; k < kNumRed; ++k) red[k] = rows > 0 ? tot[k] * mc.invV : s.red[k];
+  for (int k = 0; k < kNumRed; ++k) s.red[k] = red[k];
+  apply_update(&s, red, mc, ops);
+  *st = s;
 }
 
 // The same update for several lattices, one workgroup each.
@@ -34,14 +37,15 @@ __global__ __launch_bounds__(kUpdThreads) void k_update_batch(const UpdMember *m
   const UpdMember &u = m[blockIdx.x];
   __shared__ double tot[kNumRed];
   const int t = threadIdx.x;
-  if (u.rows > 0) {
-    sum_partials<kUpdThreads>(u.partials, u.rows, tot);
-    if (t < kNumRed) u.st->red[t] = tot[t] * u.mc.invV;
-  }
+  DevState s;
+  if (t == 0) s = *u.st;
+  if (u.rows > 0) sum_partials<kUpdThreads>(u.partials, u.rows, tot);
   if (t != 0) return;
   double red[kNumRed];
-  for (int k = 0; k < kNumRed; ++k) red[k] = u.rows > 0 ? tot[k] * u.mc.invV : u.st->red[k];
-  apply_update(u.st, red, u.mc, ops);
+  for (int k = 0; k < kNumRed; ++k) red[k] = u.rows > 0 ? tot[k] * u.mc.invV : s.red[k];
+  for (int k = 0; k < kNumRed; ++k) s.red[k] = red[k];
+  apply_update(&s, red, u.mc, ops);
+  *u.st = s;
 }
 
 // GaussLaws::checkU1 (definitions/gausslaws.h:31-53) with the charge density of
