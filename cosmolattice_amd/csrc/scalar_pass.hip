@@ -17,77 +17,11 @@
 // pair with an error-free TwoSum, so increments far below the ulp of phi are kept in lo.  Stencil
 // differences are taken on hi and lo separately.  All reductions accumulate in FP64.
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 namespace clmi {
 
 namespace {
-
-__device__ __forceinline__ double ipow_d(double v, int e)
-{
-  double r = 1.0;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-__device__ __forceinline__ float ipow_f(float v, int e)
-{
-  float r = 1.0f;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-
-// dV/dphi_n, same term and factor order as the FP64 oracle.
-template <int NF>
-__device__ __forceinline__ double dpot_d(const ScalarPoly &P, const double *phi, int n)
-{
-  double d = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    double p = P.c[t] * e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, int n)
-{
-  float d = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    float p = (float)P.c[t] * (float)e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
-{
-  float v = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    float p = (float)P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
-template <int NF>
-__device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
-{
-  double v = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    double p = P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
 
 // Error-free sum of a float pair and a float increment: returns (hi', lo') with hi'+lo' = hi+lo+d
 // up to the rounding of lo + d (Knuth TwoSum on hi and lo + d).

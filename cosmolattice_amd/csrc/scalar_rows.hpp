@@ -17,74 +17,11 @@
 #pragma once
 
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 namespace clmi {
 namespace {
 
-__device__ __forceinline__ double ipow_d(double v, int e)
-{
-  double r = 1.0;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-__device__ __forceinline__ float ipow_f(float v, int e)
-{
-  float r = 1.0f;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-template <int NF>
-__device__ __forceinline__ double dpot_d(const ScalarPoly &P, const double *phi, int n)
-{
-  double d = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    double p = P.c[t] * e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, int n)
-{
-  float d = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    float p = (float)P.c[t] * (float)e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
-{
-  float v = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    float p = (float)P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
-template <int NF>
-__device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
-{
-  double v = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    double p = P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
 __device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
 {
   const float s = lo + d;
@@ -248,8 +185,9 @@ __device__ __forceinline__ void block_reduce_rows(double (&v)[K], double *row)
 template <int NF, bool F32, int VZ, int WPR, int TY>
 using RowsTile = typename Tr<F32>::V[2][NF][TY + 2][64 * VZ * WPR];
 
-// One workgroup's work: workgroup b of the nwg that cover this lattice.
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM>
+// One workgroup's work: workgroup b of the nwg that cover this lattice.  PK: the potential's
+// compile-time exponent table (0: runtime table).
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK = 0>
 __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC, const int G, const int b,
                                           const int nwg, RowsTile<NF, F32, VZ, WPR, TY> &ldsbuf)
 {
@@ -401,23 +339,23 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
       for (int f = 0; f < NF; ++f) {
         if constexpr (!F32) {
           double p = pic[f][k];
-          if (KICK) p += a.ckick * (aK * lapd[f] - aV * dpot_d<NF>(a.poly, phid, f));
+          if (KICK) p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
           pout[f][k] = p;
           p2 += p * p;
         } else {
           float p = pic[f][k];
-          if (KICK) p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
+          if (KICK) p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
           pout[f][k] = p;
           p2f += p * p;
         }
       }
       if (RED) {
         if constexpr (!F32) {
-          vsum += pot_d<NF>(a.poly, phid);
+          vsum += pot<NF, PK, double>(a.poly, phid);
         } else {
           p2 += (double)p2f;
           g2 += (double)g2f * a.inv_dx2;
-          vsum += (double)pot_f<NF>(a.poly, phif);
+          vsum += (double)pot<NF, PK, float>(a.poly, phif);
         }
       }
     }
@@ -475,14 +413,14 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   }
 }
 
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM>
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK>
 __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
 {
   __shared__ RowsTile<NF, F32, VZ, WPR, TY> ldsbuf;
-  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, MM>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
+  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, MM, PK>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
 }
 
-template <int NF, bool F32, int VZ, int WPR, int TY, int MM>
+template <int NF, bool F32, int VZ, int WPR, int TY, int MM, int PK = 0>
 hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
                          hipStream_t s)
 {
@@ -490,7 +428,7 @@ hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &
   const dim3 blk(64 * WPR * TY);
 #define CLMI_SR(D, K, R)                                                                                       \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                             \
-    hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, MM>), dim3(blocks), blk, 0, s, a, XC, G); \
+    hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, MM, PK>), dim3(blocks), blk, 0, s, a, XC, G); \
     break;
   switch (code) {
     CLMI_SR(true, true, true)

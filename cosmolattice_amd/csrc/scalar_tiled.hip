@@ -12,76 +12,13 @@
 // workgroups of one XCD own neighbouring tiles and x-chunks, which keeps halo re-reads in that
 // XCD's L2.
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 namespace clmi {
 namespace {
 
 constexpr int TZ = 64;
 
-__device__ __forceinline__ double ipow_d(double v, int e)
-{
-  double r = 1.0;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-__device__ __forceinline__ float ipow_f(float v, int e)
-{
-  float r = 1.0f;
-  for (int i = 0; i < e; ++i) r *= v;
-  return r;
-}
-template <int NF>
-__device__ __forceinline__ double dpot_d(const ScalarPoly &P, const double *phi, int n)
-{
-  double d = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    double p = P.c[t] * e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float dpot_f(const ScalarPoly &P, const float *phi, int n)
-{
-  float d = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    const int e = P.e[t][n];
-    if (e == 0) continue;
-    float p = (float)P.c[t] * (float)e;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], f == n ? e - 1 : P.e[t][f]);
-    d += p;
-  }
-  return d;
-}
-template <int NF>
-__device__ __forceinline__ float pot_f(const ScalarPoly &P, const float *phi)
-{
-  float v = 0.0f;
-  for (int t = 0; t < P.nterms; ++t) {
-    float p = (float)P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_f(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
-template <int NF>
-__device__ __forceinline__ double pot_d(const ScalarPoly &P, const double *phi)
-{
-  double v = 0.0;
-  for (int t = 0; t < P.nterms; ++t) {
-    double p = P.c[t];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p *= ipow_d(phi[f], P.e[t][f]);
-    v += p;
-  }
-  return v;
-}
 __device__ __forceinline__ void pair_add(float &hi, float &lo, float d)
 {
   const float s = lo + d;
