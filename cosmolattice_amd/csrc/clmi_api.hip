@@ -1869,7 +1869,8 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
     c->rplan = best_rplan;
     rep = "# kernel k_scalar_rows VZ=" + std::to_string(best_rplan.VZ) + " WPR=" + std::to_string(best_rplan.WPR) +
           " TY=" + std::to_string(best_rplan.TY) + " XC=" + std::to_string(best_rplan.XC) +
-          " MM=" + std::to_string(best_rplan.MM) + "\n" + rep;
+          " MM=" + std::to_string(best_rplan.MM) +
+          " PK=" + std::to_string(scalar_rows_pk(c->cfg.n_scalars, c->f32, c->spoly, best_rplan)) + "\n" + rep;
   } else {
     c->tplan = best_plan;
     rep = "# kernel k_scalar_tiled XC=" + std::to_string(best_plan.XC) + " NT=" + std::to_string((int)best_plan.NT) +

@@ -239,6 +239,9 @@ struct RowsPlan {
   int blocks = 0, rows = 0;
 };
 bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPlan *p);
+// Potential form the full-row pass uses with plan p: 1 for the compile-time lphi4 table (FP32-bulk,
+// the shapes instantiated with it, and a run whose exponent table is lphi4's), else 0 (runtime table).
+int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p);
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,

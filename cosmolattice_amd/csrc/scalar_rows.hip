@@ -1,5 +1,6 @@
 // Shapes and launch of the full-row scalar pass (kernel in scalar_rows.hpp).
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 namespace clmi {
 
@@ -40,6 +41,16 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
   p->blocks = (N / ty) * (N / XC);
   p->rows = (N / ty) * (N / G);
   return true;
+}
+
+int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p)
+{
+  if (!f32 || poly_kind(P, nf) != 1) return 0;
+  const int v = p.VZ, w = p.WPR, t = p.TY;
+  if (nf == 1) return 1;  // every 1-field shape
+  const bool inst = (v == 2 && w == 2 && t == 8) || (v == 2 && w == 4 && t == 4) || (v == 2 && w == 1 && t == 8) ||
+                    (v == 1 && w == 1 && t == 8) || (v == 3 && w == 1 && t == 8);
+  return inst ? 1 : 0;
 }
 
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
