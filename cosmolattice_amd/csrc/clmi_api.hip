@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 using namespace clmi;
 
@@ -1488,6 +1489,7 @@ struct clmi_ensemble {
   hipGraphExec_t exec = nullptr;
   std::vector<int> captured_blocks;
   int nf = 0;
+  int pk = 0;  // potential form of the batched passes (1: every member has the compile-time lphi4 table)
   bool f32 = false, ex = false;
   std::string err;
 };
@@ -1588,7 +1590,7 @@ int ens_tables(clmi_ensemble *e)
 // One steady-state leapfrog step of a group: fused drift+kick pass, then the background update.
 int ens_step(clmi_ensemble *e, const clmi_ensemble::Group &g, int parity, hipStream_t st)
 {
-  EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, g.vz, true, true, e->ex, e->d_rows[parity] + g.first, g.count,
+  EHIP(e, launch_scalar_rows_batch(e->nf, e->f32, e->pk, g.vz, true, true, e->ex, e->d_rows[parity] + g.first, g.count,
                                    g.blocks, st));
   if (e->ex)
     EHIP(e, launch_update_batch(e->d_upd + g.first, g.count,
@@ -1655,6 +1657,9 @@ int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, cl
   e->device = c0->device;
   e->nf = c0->cfg.n_scalars;
   e->f32 = c0->f32;
+  e->pk = e->f32 ? 1 : 0;
+  for (clmi_ctx *c : e->m)
+    if (poly_kind(c->spoly, e->nf) != 1) e->pk = 0;
   e->ex = c0->cfg.expansion != 0;
   e->group_bytes = group_bytes ? group_bytes : SIZE_MAX;
   int rc = CLMI_OK;
@@ -1726,7 +1731,7 @@ int clmi_ensemble_autotune(clmi_ensemble *e, int reps, char *report, size_t repo
       if ((rc = ens_tables(e)) != CLMI_OK) break;
       const clmi_ensemble::Group &g = e->groups[gi];
       auto launch = [&]() {
-        return launch_scalar_rows_batch(e->nf, e->f32, g.vz, true, true, e->ex, e->d_rows[0] + g.first, g.count,
+        return launch_scalar_rows_batch(e->nf, e->f32, e->pk, g.vz, true, true, e->ex, e->d_rows[0] + g.first, g.count,
                                         g.blocks, e->stream);
       };
       hipError_t he = launch();

@@ -189,7 +189,8 @@ struct RowsMember {   // one lattice's fused pass inside a batched launch (devic
   int vz;             // row shape: VZ = N / 64 (WPR = 1, TY = 8)
   int pad;
 };
-hipError_t launch_scalar_rows_batch(int nf, bool f32, int vz, bool drift, bool kick, bool red,
+// pk: potential form of every member (1: compile-time lphi4 table, FP32-bulk only; see scalar_rows_pk).
+hipError_t launch_scalar_rows_batch(int nf, bool f32, int pk, int vz, bool drift, bool kick, bool red,
                                     const RowsMember *d_members, int n, int total_blocks, hipStream_t s);
 struct UpdMember {    // one lattice's background update inside a batched launch
   DevState *st;
