@@ -26,9 +26,23 @@
 
 namespace clmi {
 
+namespace {
+hipError_t fill_bin_table(BinTable &t, int N, double dk, int nbins);
+}
+
+// Builds the table for (N, dk, nbins); on failure the table is left empty (N = 0), so the next call
+// builds it again.
 hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins)
 {
   free_bin_table(t);
+  const hipError_t e = fill_bin_table(t, N, dk, nbins);
+  if (e != hipSuccess) free_bin_table(t);
+  return e;
+}
+
+namespace {
+hipError_t fill_bin_table(BinTable &t, int N, double dk, int nbins)
+{
   const uint32_t Nh = N / 2 + 1;
   const uint32_t M = (uint32_t)N * N * Nh;
   const int half = N / 2;
@@ -115,6 +129,7 @@ hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins)
   if (e != hipSuccess) return e;
   return hipMemcpy(t.d_start, start.data(), sizeof(uint32_t) * (size_t)(nbins + 1), hipMemcpyHostToDevice);
 }
+}  // namespace
 
 void free_bin_table(BinTable &t)
 {
