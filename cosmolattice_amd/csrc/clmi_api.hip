@@ -1019,6 +1019,8 @@ int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, in
   }
   if (c->bin_out_n < nbins) {
     if (c->bin_out) HIPCHK(c, hipFree(c->bin_out));
+    c->bin_out = nullptr;
+    c->bin_out_n = 0;
     HIPCHK(c, hipMalloc(&c->bin_out, sizeof(double) * nbins));
     c->bin_out_n = nbins;
   }
