@@ -520,12 +520,14 @@ def test_lphi4u1_gauss_conserved_fp64_and_f32(gpu):
 
 def test_spectrum_bins_parity(gpu, oracle_mod):
     """hipFFT R2C + deterministic shell binning equals the numpy restatement within 1e-12, and a
-    repeated call is bit-identical.  The cases cover the streaming pass with four and with one LDS
+    repeated call is bit-identical.  The cases cover the streaming pass at the production size (256^3,
+    8.4 M modes, 1024 workgroups) and with four and with one LDS
     histogram per workgroup (nbins <= 2048 and 2048 < nbins <= 8192), small N (lines shorter than a
     wave: up to 5 and 14 lines per 64-mode chunk) and the bin-sorted table used above 8192 bins."""
     say("case spectrum")
     spec = bench_spec()
     cases = ((32, 1.0, (CLMI_F64, CLMI_F32_BULK)), (24, 0.5, (CLMI_F64, CLMI_F32_BULK)), (128, 1.0, (CLMI_F64,)),
+             (256, 1.0, (CLMI_F32_BULK,)),
              (30, 0.7, (CLMI_F64,)), (8, 0.4, (CLMI_F64,)), (24, 0.01, (CLMI_F64,)), (24, 0.002, (CLMI_F64,)))
     for N, dk, precs in cases:
         rng = np.random.default_rng(N)
