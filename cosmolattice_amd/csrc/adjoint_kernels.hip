@@ -176,13 +176,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_kick(const AdjArgs a, double c, 
   block_reduce_store<kAdjRed>(acc, a.partials + (size_t)blockIdx.x * kAdjRed);
 }
 
-// Fixed-order sum of `rows` partial rows into out[0..kAdjRed).
+// Fixed-order sum of `rows` partial rows into out[0..kAdjRed): thread (g, k) adds slot k of rows g,
+// g + NG, ... (coalesced across a wavefront, eight loads in flight), then the NG group sums are added
+// in group order.
 __global__ __launch_bounds__(kBlock) void k_adj_finish(const double *partials, int rows, double *out)
 {
+  static_assert(kBlock % kAdjRed == 0, "row groups");
+  constexpr int NG = kBlock / kAdjRed;
+  __shared__ double sh[NG][kAdjRed];
+  const int k = threadIdx.x % kAdjRed, g = threadIdx.x / kAdjRed;
+  double s = 0.0;
+#pragma unroll 8
+  for (int r = g; r < rows; r += NG) s += partials[(size_t)r * kAdjRed + k];
+  sh[g][k] = s;
+  __syncthreads();
   if (threadIdx.x < kAdjRed) {
-    double s = 0.0;
-    for (int r = 0; r < rows; ++r) s += partials[(size_t)r * kAdjRed + threadIdx.x];
-    out[threadIdx.x] = s;
+    double t = 0.0;
+    for (int q = 0; q < NG; ++q) t += sh[q][threadIdx.x];
+    out[threadIdx.x] = t;
   }
 }
 

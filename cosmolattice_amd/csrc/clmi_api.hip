@@ -453,14 +453,19 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
   c->blocks_s = scalar_pass_blocks(c->V);
   c->blocks_g = gauge_pass_blocks(c->V);
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
+    // Default x-chunk of the full-row pass: the longest of 32, 16, 8 that still gives every CU a
+    // workgroup, else the shortest (small lattices).  clmi_autotune refines it; results do not depend on it.
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu < 1)
+      ncu = 256;
     RowsPlan rp;
     const int mm_req = (g.flags >> 8) & 15;
-    for (int xc : {32, 16, 8, 4})
+    for (int xc : {32, 16, 8})
       if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, mm_req ? mm_req : kDefaultMemMode,
                            rows_shape(g.flags), &rp)) {
         c->rowk = true;
         c->rplan = rp;
-        break;
+        if (rp.blocks >= ncu) break;
       }
     if (mm_req && !c->rowk) {
       fail(c, CLMI_E_INVALID, "memory mode " + std::to_string(mm_req) + " is not available for this lattice");
