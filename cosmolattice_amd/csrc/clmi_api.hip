@@ -1664,7 +1664,7 @@ int clmi_ensemble_create(clmi_ctx *const *members, int n, size_t group_bytes, cl
   e->device = c0->device;
   e->nf = c0->cfg.n_scalars;
   e->f32 = c0->f32;
-  e->pk = e->f32 ? 1 : 0;
+  e->pk = 1;  // the batched shapes are the 512-thread ones, where both precisions use it
   for (clmi_ctx *c : e->m)
     if (poly_kind(c->spoly, e->nf) != 1) e->pk = 0;
   e->ex = c0->cfg.expansion != 0;

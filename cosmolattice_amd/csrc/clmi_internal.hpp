@@ -189,7 +189,7 @@ struct RowsMember {   // one lattice's fused pass inside a batched launch (devic
   int vz;             // row shape: VZ = N / 64 (WPR = 1, TY = 8)
   int pad;
 };
-// pk: potential form of every member (1: compile-time lphi4 table, FP32-bulk only; see scalar_rows_pk).
+// pk: potential form of every member (1: compile-time lphi4 table; see scalar_rows_pk).
 hipError_t launch_scalar_rows_batch(int nf, bool f32, int pk, int vz, bool drift, bool kick, bool red,
                                     const RowsMember *d_members, int n, int total_blocks, hipStream_t s);
 struct UpdMember {    // one lattice's background update inside a batched launch
@@ -240,8 +240,9 @@ struct RowsPlan {
   int blocks = 0, rows = 0;
 };
 bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPlan *p);
-// Potential form the full-row pass uses with plan p: 1 for the compile-time lphi4 table (FP32-bulk,
-// the shapes instantiated with it, and a run whose exponent table is lphi4's), else 0 (runtime table).
+// Potential form the full-row pass uses with plan p: 1 for the compile-time lphi4 table (a run whose
+// exponent table is lphi4's, on the shapes instantiated with it: FP32-bulk main shapes, FP64 512-thread
+// shapes), else 0 (runtime table).
 int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p);
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s);

@@ -45,8 +45,9 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
 
 int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p)
 {
-  if (!f32 || poly_kind(P, nf) != 1) return 0;
+  if (poly_kind(P, nf) != 1) return 0;
   const int v = p.VZ, w = p.WPR, t = p.TY;
+  if (!f32) return (w == 1 && t == 8 && v <= 3) ? 1 : 0;  // FP64: the 512-thread shapes
   if (nf == 1) return 1;  // every 1-field shape
   const bool inst = (v == 2 && w == 2 && t == 8) || (v == 2 && w == 4 && t == 4) || (v == 2 && w == 1 && t == 8) ||
                     (v == 1 && w == 1 && t == 8) || (v == 3 && w == 1 && t == 8);

@@ -59,12 +59,16 @@ hipError_t launch_scalar_rows_batch(int nf, bool f32, int pk, int vz, bool drift
                                     const RowsMember *d_members, int n, int total_blocks, hipStream_t s)
 {
   if (nf == 1) {
-    if (!f32) return launch_batch_vz<1, false, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
+    if (!f32)
+      return pk == 1 ? launch_batch_vz<1, false, 1>(vz, drift, kick, red, d_members, n, total_blocks, s)
+                     : launch_batch_vz<1, false, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
     return pk == 1 ? launch_batch_vz<1, true, 1>(vz, drift, kick, red, d_members, n, total_blocks, s)
                    : launch_batch_vz<1, true, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
   }
   if (nf == 2) {
-    if (!f32) return launch_batch_vz<2, false, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
+    if (!f32)
+      return pk == 1 ? launch_batch_vz<2, false, 1>(vz, drift, kick, red, d_members, n, total_blocks, s)
+                     : launch_batch_vz<2, false, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
     return pk == 1 ? launch_batch_vz<2, true, 1>(vz, drift, kick, red, d_members, n, total_blocks, s)
                    : launch_batch_vz<2, true, 0>(vz, drift, kick, red, d_members, n, total_blocks, s);
   }

@@ -5,23 +5,34 @@ namespace clmi {
 
 hipError_t launch_scalar_rows_f64_nf2(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p, hipStream_t s)
 {
-#define CLMI_SHAPE(NF_, F32_, VZ_, WPR_, TY_)                                                          \
-  if (p.VZ == VZ_ && p.WPR == WPR_ && p.TY == TY_)                                                     \
-    switch (p.MM) {                                                                                    \
-      case 1: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
-      case 7: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 7>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
-      case 3: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 3>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
-      case 15: return dispatch_rows<NF_, F32_, VZ_, WPR_, TY_, 15>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
-      default: return hipErrorInvalidValue;                                                            \
-    }
-  CLMI_SHAPE(2, false, 2, 2, 8)
-  CLMI_SHAPE(2, false, 4, 1, 8)
-  CLMI_SHAPE(2, false, 2, 1, 8)
-  CLMI_SHAPE(2, false, 1, 1, 8)
-  CLMI_SHAPE(2, false, 3, 1, 8)
-  CLMI_SHAPE(2, false, 2, 4, 4)
-  CLMI_SHAPE(2, false, 2, 2, 4)
-  CLMI_SHAPE(2, false, 4, 1, 4)
+  // The lphi4 potential has a compile-time form for the 512-thread shapes (64^3 to 192^3); the
+  // 1024-thread shapes are at the 128-VGPR cap, where it would add spills.
+  const int pk = scalar_rows_pk(2, false, a.poly, p);
+#define CLMI_MM(NF_, VZ_, WPR_, TY_, PK_)                                                                \
+  switch (p.MM) {                                                                                      \
+    case 1: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 1, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
+    case 7: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 7, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
+    case 3: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 3, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
+    case 15: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 15, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+    default: return hipErrorInvalidValue;                                                              \
+  }
+#define CLMI_SHAPE(NF_, VZ_, WPR_, TY_)                                                                  \
+  if (p.VZ == VZ_ && p.WPR == WPR_ && p.TY == TY_) CLMI_MM(NF_, VZ_, WPR_, TY_, 0)
+#define CLMI_SHAPE_PK(NF_, VZ_, WPR_, TY_)                                                               \
+  if (p.VZ == VZ_ && p.WPR == WPR_ && p.TY == TY_) {                                                   \
+    if (pk == 1) CLMI_MM(NF_, VZ_, WPR_, TY_, 1)                                                        \
+    CLMI_MM(NF_, VZ_, WPR_, TY_, 0)                                                                     \
+  }
+  CLMI_SHAPE(2, 2, 2, 8)
+  CLMI_SHAPE(2, 4, 1, 8)
+  CLMI_SHAPE_PK(2, 2, 1, 8)
+  CLMI_SHAPE_PK(2, 1, 1, 8)
+  CLMI_SHAPE_PK(2, 3, 1, 8)
+  CLMI_SHAPE(2, 2, 4, 4)
+  CLMI_SHAPE(2, 2, 2, 4)
+  CLMI_SHAPE(2, 4, 1, 4)
+#undef CLMI_SHAPE_PK
+#undef CLMI_MM
 #undef CLMI_SHAPE
   return hipErrorInvalidValue;
 }
