@@ -66,8 +66,14 @@ __device__ __forceinline__ void vload(const S *pg, uint32_t s, S (&o)[VZ])
   } else if constexpr (VZ == 3 && sizeof(S) == 4) {
     // one 12-byte access (global_load_dwordx3), aligned to the element only (s is a multiple of 3).
     // Only for 4-byte elements: a 3-element vector of doubles is accessed as 32 bytes.
+    // (loaded here, not through ldq, so that the pointer keeps the element alignment)
     typedef S v3 __attribute__((ext_vector_type(3), aligned(sizeof(S))));
-    const v3 v = ldq<NTL, v3>((cgptr<v3>)(p + s));
+    const cgptr<v3> q = (cgptr<v3>)(p + s);
+    v3 v;
+    if constexpr (NTL)
+      v = __builtin_nontemporal_load(q);
+    else
+      v = *q;
     o[0] = v.x; o[1] = v.y; o[2] = v.z;
   } else {
 #pragma unroll
