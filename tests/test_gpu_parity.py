@@ -551,7 +551,8 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
 def test_norm_averages_and_magnetic_spectrum(gpu, oracle_mod, prec):
     """<|v|>, <|v|^2> of the complex scalar, its momentum, the U(1) momenta and the magnetic field
     equal numpy on the lattice's own arrays; <|B|^2> equals the evolver's field-strength reduction;
-    the spectrum of each B component equals the oracle's binning of numpy's B."""
+    the spectrum of each B component equals the oracle's binning of numpy's B, and those of the complex
+    scalar's and the gauge field's components equal the oracle's binning of the lattice's arrays."""
     say("case norms", prec)
     spec = u1_spec()
     N, dx = 32, 0.7
@@ -577,6 +578,12 @@ def test_norm_averages_and_magnetic_spectrum(gpu, oracle_mod, prec):
     for c in range(3):
         s, cnt, _ = lat.spectrum_bins(MAG_U1, 0, c, 1.0)
         so, co, _ = oracle_mod.spectrum_bins(B[c], 1.0)
+        np.testing.assert_array_equal(cnt, co)
+        np.testing.assert_allclose(s, so, rtol=1e-11, atol=1e-12 * so.max())
+    # spectra of the complex scalar's and the gauge momenta's components
+    for kind, c in ((FLD_CS, 0), (FLD_CS, 1), (PI_CS, 1), (FLD_U1, 2), (PI_U1, 0)):
+        s, cnt, _ = lat.spectrum_bins(kind, 0, c, 1.0)
+        so, co, _ = oracle_mod.spectrum_bins(got[(kind, c)], 1.0)
         np.testing.assert_array_equal(cnt, co)
         np.testing.assert_allclose(s, so, rtol=1e-11, atol=1e-12 * so.max())
     with pytest.raises(Exception):
