@@ -276,6 +276,40 @@ def test_host_lphi4u1_measurement_files(gpu, tmp_path):
         assert np.all(np.isfinite(rows)) and rows[:, 1].max() > 0
 
 
+@pytest.mark.parametrize("version", [1, 2])
+def test_host_spectrum_normalisation_parseval(gpu, tmp_path, version):
+    """PS_type 1 spectra written by the lphi4 program obey Parseval: with n_b the bin's |n| used in the
+    prefactor (b dk for PS_version 2, the mean |n| of the bin's modes for PS_version 1),
+    sum_{b >= 1} PS_b dk / n_b = <f^2> - <f>^2, which the same program writes as rms^2 in
+    average_scalar_<i>.txt (and a^(2 alpha - 6) times the momentum variance for PS(pi)).  This pins the
+    spectrum prefactor of measurements/powerspectrum.h:85-110 for both versions."""
+    say(f"case host spectrum parseval v{version}")
+    N, dx, dt, steps, every = 32, 0.6, 0.005, 4, 2
+    extra = f"PS_type = 1\nPS_version = {version}\nbaseSeed = 5\nkCutOff = 2.0\n"
+    run_program("lphi4", tmp_path, lphi4_params(N, dx, dt, steps, every, extra=extra))
+    text = (tmp_path / "spectra_scalar_1.txt").read_text()
+    blocks = [np.loadtxt(b.splitlines()) for b in
+              ["\n".join(ln for ln in blk.splitlines() if not ln.startswith("#")) for blk in text.split("\n\n")]
+              if b.strip()]
+    means = np.atleast_2d(np.loadtxt(tmp_path / "average_scalar_1.txt"))
+    assert len(blocks) == means.shape[0] >= 2
+    nb = blocks[0].shape[0]
+    kIR = 2 * math.pi / (N * dx)
+    if version == 2:
+        nbar = np.arange(nb) * 1.0
+    else:
+        n = np.fft.fftfreq(N, d=1.0 / N)
+        n = np.where(n == -N // 2, N // 2, n)
+        nn = np.sqrt(n[:, None, None] ** 2 + n[None, :, None] ** 2 + n[None, None, :] ** 2).ravel()
+        b = np.minimum(np.floor(nn + 0.5).astype(int), nb - 1)
+        nbar = np.bincount(b, weights=nn, minlength=nb) / np.maximum(np.bincount(b, minlength=nb), 1)
+    for blk, row in zip(blocks, means):
+        np.testing.assert_allclose(blk[:, 0], np.arange(nb) * kIR, rtol=1e-12)
+        for col, rms in ((1, row[5]), (2, row[6])):
+            total = np.sum(blk[1:, col] / nbar[1:])
+            assert total == pytest.approx(rms ** 2, rel=1e-7), (col, total, rms ** 2)
+
+
 def test_host_backup_and_restart(gpu, tmp_path):
     """A run restarted from its own backup (load_file, as SimulationManager::loadSim) ends where the
     uninterrupted run ends, to rounding: the restart recomputes the cached averages
