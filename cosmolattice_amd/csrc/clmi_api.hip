@@ -1001,23 +1001,27 @@ int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, in
   if (!(dk > 0) || nbins != clmi_spectrum_nbins(c, dk) || !sum_f2)
     return fail(c, CLMI_E_INVALID, "nbins must equal clmi_spectrum_nbins(ctx, dk)");
   CHK(flush(c));
+  double *fft_in = nullptr;
   if (kind == CLMI_MAG_U1) {
     if (comp < 0 || comp > 2) return fail(c, CLMI_E_INVALID, "CLMI_MAG_U1: comp must be 0, 1 or 2");
     NormArgs a{};
     CHK(u1_curl_args(c, index, &a));
     CHK(ensure_fft(c, true, false, false));
     HIPCHK(c, launch_curl_component(a, comp, c->stage, c->stream));
+    fft_in = c->stage;
   } else {
     DevArr *arr = nullptr;
     bool is_field = false;
     CHK(select_arr(c, kind, index, comp, &arr, &is_field));
     CHK(ensure_fft(c, true, false, false));
-    if (c->f32)
+    if (c->f32) {
       HIPCHK(c, launch_join_f64((const float *)arr->p, arr->lo, c->stage, c->V, c->stream));
-    else
-      HIPCHK(c, hipMemcpyAsync(c->stage, arr->p, sizeof(double) * c->V, hipMemcpyDeviceToDevice, c->stream));
+      fft_in = c->stage;
+    } else {
+      fft_in = (double *)arr->p;  // the out-of-place R2C transform reads the array and leaves it unchanged
+    }
   }
-  FFTCHK(c, hipfftExecD2Z(c->plan_r2c, (hipfftDoubleReal *)c->stage, (hipfftDoubleComplex *)c->fk));
+  FFTCHK(c, hipfftExecD2Z(c->plan_r2c, (hipfftDoubleReal *)fft_in, (hipfftDoubleComplex *)c->fk));
   if (c->bins.nbins != nbins || c->bins.dk != dk || c->bins.N != c->N) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, build_bin_table(c->bins, c->N, dk, nbins));

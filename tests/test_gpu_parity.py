@@ -582,7 +582,7 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
     say("case spectrum")
     spec = bench_spec()
     cases = ((32, 1.0, (CLMI_F64, CLMI_F32_BULK)), (24, 0.5, (CLMI_F64, CLMI_F32_BULK)), (128, 1.0, (CLMI_F64,)),
-             (256, 1.0, (CLMI_F32_BULK,)),
+             (256, 1.0, (CLMI_F32_BULK, CLMI_F64)),
              (30, 0.7, (CLMI_F64,)), (8, 0.4, (CLMI_F64,)), (24, 0.01, (CLMI_F64,)), (24, 0.002, (CLMI_F64,)))
     for N, dk, precs in cases:
         rng = np.random.default_rng(N)
@@ -593,6 +593,8 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
             lat.upload(FLD_S, 1, 0, f)
             s, c, m = lat.spectrum_bins(FLD_S, 1, 0, dk)
             s2, _, _ = lat.spectrum_bins(FLD_S, 1, 0, dk)
+            # the transform reads the field array itself in FP64: it must come back unchanged
+            assert np.array_equal(lat.download(FLD_S, 1, 0), f) or prec == CLMI_F32_BULK
             so, co, mo = oracle_mod.spectrum_bins(lat.download(FLD_S, 1, 0), dk)
             np.testing.assert_array_equal(s, s2)
             np.testing.assert_array_equal(c, co)
@@ -600,6 +602,25 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
             np.testing.assert_allclose(m, mo, rtol=1e-12)
             np.testing.assert_allclose(s, so, rtol=1e-12, atol=1e-12 * so.max())
             lat.close()
+
+
+def test_spectrum_leaves_fp64_field_unchanged_512(gpu):
+    """In FP64 the spectrum's R2C transform reads the field array itself (out of place): at 512^3, where
+    hipFFT plans differently from the smaller sizes, the field comes back bit-identical and a repeated
+    spectrum is bit-identical."""
+    say("case spectrum 512 fp64 input preserved")
+    spec = bench_spec()
+    N = 512
+    f = np.random.default_rng(512).normal(size=(N, N, N))
+    lat = Lattice(spec, N, 0.5, 0.01, CLMI_F64)
+    lat.upload(FLD_S, 1, 0, f)
+    s1, _, _ = lat.spectrum_bins(FLD_S, 1, 0, 1.0)
+    s2, _, _ = lat.spectrum_bins(FLD_S, 1, 0, 1.0)
+    assert np.array_equal(lat.download(FLD_S, 1, 0), f)
+    assert np.array_equal(s1, s2)
+    # Parseval: the bins hold the full N^3 mode set
+    assert s1.sum() == pytest.approx(N ** 3 * np.sum(f * f), rel=1e-11)
+    lat.close()
 
 
 @pytest.mark.parametrize("prec", [CLMI_F64, CLMI_F32_BULK])
