@@ -59,3 +59,23 @@ def test_product_path_does_not_import_oracle():
     for p in (ROOT / "cosmolattice_amd").rglob("*"):
         if p.suffix in (".py", ".hip", ".hpp", ".h", ".cpp", ".txt"):
             assert not pat.search(p.read_text()), p
+
+
+def test_binding_constants_match_the_header():
+    """Every plain integer #define of include/clmi.h that the Python binding mirrors has the same value
+    there (field kinds under the binding's short names, flags, precisions, limits)."""
+    from cosmolattice_amd import clmi
+
+    text = (ROOT / "include" / "clmi.h").read_text()
+    defines = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define\s+(CLMI_\w+)\s+(-?\d+)\s*$", text, re.M)}
+    short = {"CLMI_FLD_S": "FLD_S", "CLMI_PI_S": "PI_S", "CLMI_FLD_CS": "FLD_CS", "CLMI_PI_CS": "PI_CS",
+             "CLMI_FLD_U1": "FLD_U1", "CLMI_PI_U1": "PI_U1", "CLMI_MAG_U1": "MAG_U1"}
+    checked = 0
+    for name, value in defines.items():
+        attr = short.get(name, name)
+        if hasattr(clmi, attr):
+            assert getattr(clmi, attr) == value, name
+            checked += 1
+    assert checked >= 12
+    assert clmi.CLMI_FLAG_MEMMODE(15) == 15 << 8
+
