@@ -327,6 +327,45 @@ def test_primitives_fp64_parity(gpu, oracle_mod):
     assert lat.scale_factor_kernel() == pytest.approx(orc.scale_factor_kernel(), rel=1e-13)
 
 
+def test_f32_bulk_tracks_fp64_at_256(gpu):
+    """The headline workload at its full size: 2-field lphi4 at 256^3 (q = 100, kIR = 0.05, dt = 0.005,
+    self-consistent expansion) from the same seeded vacuum initial conditions in FP32-bulk and FP64.
+    After 400 leapfrog steps the scale factors agree to 1e-8, both Friedmann residuals stay small and
+    agree, and the chi spectrum in the band k in [0.5, 1.5] agrees to 1e-3 on average."""
+    say("case f32_bulk vs fp64 at 256^3")
+    spec = lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+    N, kIR, dt = 256, 0.05, 0.005
+    dx = 2 * math.pi / kIR / N
+    out = {}
+    for prec in (CLMI_F64, CLMI_F32_BULK):
+        lat = Lattice(spec, N, dx, dt, prec, expansion=True)
+        kin = sum(0.5 * p * p for p in spec.pi0)
+        aDot0 = spec.fStar / spec.MPl * math.sqrt((kin + spec.potential(spec.fld0)) / 3.0)
+        lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=aDot0)
+        m2 = spec.mass2(spec.fld0)
+        for f in range(2):
+            lat.grf_scalar(f, 4242, f, m2[f], aDot0, kIR, N * kIR / 4, spec.fld0[f], spec.pi0[f])
+        _, _, rhs = lat.hubble_constraint()
+        lat.set_background(aDotI=math.sqrt(rhs))
+        lat.lf_init()
+        lat.lf_evolve(400)
+        lat.lf_sync()
+        hc = lat.hubble_constraint()[0]
+        s, c, _ = lat.spectrum_bins(FLD_S, 1, 0, 1.0)
+        out[prec] = (lat.background()["aI"], hc, s / np.maximum(c, 1))
+        lat.close()
+    a64, h64, p64 = out[CLMI_F64]
+    a32, h32, p32 = out[CLMI_F32_BULK]
+    assert a64 > 1.0
+    assert abs(a32 - a64) / a64 < 1e-8
+    assert abs(h64) < 1e-3 and abs(h32) < 1e-3
+    assert abs(h32 - h64) < 1e-5
+    n = np.arange(len(p64))
+    band = (n * kIR >= 0.5) & (n * kIR <= 1.5)
+    err = np.abs(p32[band] - p64[band]) / p64[band]
+    assert err.mean() < 1e-3, err.mean()
+
+
 def test_free_plane_wave_closed_form_256(gpu):
     """Full-size check at 256^3: a free massive plane wave follows phi_n = A cos(n W dt) exactly
     (size-independent property of the leapfrog recurrence), in FP64 and FP32-bulk."""
