@@ -90,7 +90,8 @@ __device__ __forceinline__ void store_out(S *p, uint32_t i, S v)
 
 // Partial sums are written per (group of G planes, y-tile, z-tile), independent of the x-chunk
 // length XC (a multiple of G), so every XC gives the same summation order.
-template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY, bool NT>
+// PK: the potential's compile-time exponent table (0: runtime table).
+template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY, bool NT, int PK>
 __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a, const int XC, const int G)
 {
   using T = Tr<F32>;
@@ -242,14 +243,14 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
         if constexpr (!F32) {
           double p = pic[r][f];
           if (KICK) {
-            p += a.ckick * (aK * lapd[f] - aV * dpot_d<NF>(a.poly, phid, f));
+            p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
             store_out<NT>((double *)a.pi_out[f], s, p);
           }
           p2 += p * p;
         } else {
           float p = pic[r][f];
           if (KICK) {
-            p += kK * lapf[f] - kV * dpot_f<NF>(a.poly, phif, f);
+            p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
             store_out<NT>((float *)a.pi_out[f], s, p);
           }
           p2f += p * p;
@@ -259,11 +260,11 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
         if constexpr (!F32) {
           acc0 += p2;
           acc1 += g2;
-          acc2 += pot_d<NF>(a.poly, phid);
+          acc2 += pot<NF, PK, double>(a.poly, phid);
         } else {
           acc0 += (double)p2f;
           acc1 += (double)g2f * a.inv_dx2;
-          acc2 += (double)pot_f<NF>(a.poly, phif);
+          acc2 += (double)pot<NF, PK, float>(a.poly, phif);
         }
       }
     }
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
   }
 }
 
-template <int NF, bool F32, bool NT>
+template <int NF, bool F32, bool NT, int PK>
 hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
                           hipStream_t s)
 {
@@ -311,7 +312,7 @@ hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs 
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
 #define CLMI_ST(D, K, R)                                                                                        \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                              \
-    hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY, NT>), dim3(blocks), dim3(kBlock), 0, s, a, XC, G); \
+    hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY, NT, PK>), dim3(blocks), dim3(kBlock), 0, s, a, XC, G); \
     break;
   switch (code) {
     CLMI_ST(true, true, true)
@@ -350,16 +351,22 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                                const TiledPlan &p, hipStream_t s)
 {
-  const int code = (nf == 2 ? 4 : 0) | (f32 ? 2 : 0) | (p.NT ? 1 : 0);
+  // FP32-bulk uses the compile-time lphi4 potential when the run's exponent table is lphi4's
+  const bool pk = f32 && poly_kind(a.poly, nf) == 1;
+  const int code = (nf == 2 ? 4 : 0) | (f32 ? 2 : 0) | (p.NT ? 1 : 0) | (pk ? 8 : 0);
   switch (code) {
-    case 0: return dispatch_tiled<1, false, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 1: return dispatch_tiled<1, false, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 2: return dispatch_tiled<1, true, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 3: return dispatch_tiled<1, true, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 4: return dispatch_tiled<2, false, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 5: return dispatch_tiled<2, false, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 6: return dispatch_tiled<2, true, false>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 7: return dispatch_tiled<2, true, true>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 0: return dispatch_tiled<1, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 1: return dispatch_tiled<1, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 2: return dispatch_tiled<1, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 3: return dispatch_tiled<1, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 4: return dispatch_tiled<2, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 5: return dispatch_tiled<2, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 6: return dispatch_tiled<2, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 7: return dispatch_tiled<2, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 10: return dispatch_tiled<1, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 11: return dispatch_tiled<1, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 14: return dispatch_tiled<2, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 15: return dispatch_tiled<2, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
     default: return hipErrorInvalidValue;
   }
 }
