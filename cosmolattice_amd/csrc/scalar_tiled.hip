@@ -351,8 +351,8 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                                const TiledPlan &p, hipStream_t s)
 {
-  // FP32-bulk uses the compile-time lphi4 potential when the run's exponent table is lphi4's
-  const bool pk = f32 && poly_kind(a.poly, nf) == 1;
+  // the compile-time lphi4 potential when the run's exponent table is lphi4's
+  const bool pk = poly_kind(a.poly, nf) == 1;
   const int code = (nf == 2 ? 4 : 0) | (f32 ? 2 : 0) | (p.NT ? 1 : 0) | (pk ? 8 : 0);
   switch (code) {
     case 0: return dispatch_tiled<1, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
@@ -367,6 +367,10 @@ hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red
     case 11: return dispatch_tiled<1, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
     case 14: return dispatch_tiled<2, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
     case 15: return dispatch_tiled<2, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 12: return dispatch_tiled<2, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 13: return dispatch_tiled<2, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 8: return dispatch_tiled<1, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 9: return dispatch_tiled<1, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
     default: return hipErrorInvalidValue;
   }
 }
