@@ -274,12 +274,18 @@ def measured_traffic(kernel: str, f32: bool):
     FETCH_SIZE / WRITE_SIZE summary of the same workload (calibrated against a copy kernel), or None
     when that summary has no row for this kernel instantiation."""
     import re
-    m = re.search(r"VZ=(\d+) WPR=(\d+) TY=(\d+) XC=\d+ MM=(\d+) PK=(\d+)", kernel)
     path = Path(__file__).resolve().parent / TRAFFIC_FILE
-    if not m or not path.exists():
+    m = re.search(r"VZ=(\d+) WPR=(\d+) TY=(\d+) XC=\d+ MM=(\d+) PK=(\d+)", kernel)
+    t = re.search(r"k_scalar_tiled XC=\d+ NT=(\d+) PK=(\d+)", kernel)
+    if not (m or t) or not path.exists():
         return None
-    vz, wpr, ty, mm, pk = m.groups()
-    name = f"`k_scalar_rows<2, {'true' if f32 else 'false'}, {vz}, {wpr}, {ty}, true, true, true, {mm}, {pk}>`"
+    fl = "true" if f32 else "false"
+    if m:
+        vz, wpr, ty, mm, pk = m.groups()
+        name = f"`k_scalar_rows<2, {fl}, {vz}, {wpr}, {ty}, true, true, true, {mm}, {pk}>`"
+    else:
+        nt, pk = t.groups()
+        name = f"`k_scalar_tiled<2, {fl}, true, true, true, 4, {'true' if nt == '1' else 'false'}, {pk}>`"
     for ln in path.read_text().splitlines():
         cells = [c.strip() for c in ln.strip().strip("|").split("|")]
         if len(cells) == 4 and cells[0] == name:

@@ -460,8 +460,14 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
       ncu = 256;
     RowsPlan rp;
     const int mm_req = (g.flags >> 8) & 15;
+    // FP64 at 256^3: the full-row pass needs 1024-thread workgroups, whose 128-VGPR cap makes it spill, and
+    // the tiled pass is faster there.  Batched ensembles stop at 192^3, so nothing needs the row plan.  A
+    // memory-mode or shape flag still asks for the full-row pass.
+    const bool asked = mm_req != 0 || (g.flags & (CLMI_FLAG_SHAPE_ALT | CLMI_FLAG_SHAPE_TY4)) != 0;
+    const bool prefer_tiled = g.precision == CLMI_F64 && g.N == 256 && !asked;
     for (int xc : {32, 16, 8})
-      if (scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, mm_req ? mm_req : kDefaultMemMode,
+      if (!prefer_tiled &&
+          scalar_rows_plan(g.N, g.n_scalars, g.precision == CLMI_F32_BULK, xc, mm_req ? mm_req : kDefaultMemMode,
                            rows_shape(g.flags), &rp)) {
         c->rowk = true;
         c->rplan = rp;
@@ -1890,7 +1896,7 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   } else {
     c->tplan = best_plan;
     rep = "# kernel k_scalar_tiled XC=" + std::to_string(best_plan.XC) + " NT=" + std::to_string((int)best_plan.NT) +
-          "\n" + rep;
+          " PK=" + std::to_string(poly_kind(c->spoly, c->cfg.n_scalars)) + "\n" + rep;
   }
   if (report && report_len) {
     std::strncpy(report, rep.c_str(), report_len - 1);
