@@ -55,7 +55,8 @@ extern "C" {
 /* clmi_config.flags: memory mode m of the full-row lattice pass (0 = the library's default, 15; 1 =
  * non-temporal stores, 3 = also non-temporal loads of the momenta, 7 = also of the workgroup's own
  * field rows, 15 = as 7 but ordinary loads for the tile's first and last rows).  Results do not
- * depend on it; clmi_autotune chooses one. */
+ * depend on it; clmi_autotune chooses one.  This flag, like the shape flags, also selects the
+ * full-row pass where another pass is the default (FP64 at 256^3 runs the tiled pass). */
 #define CLMI_FLAG_MEMMODE(m) ((m) << 8)
 
 /* Field kinds, numbered as the reference's FieldsNumbering (fieldsnumbering.h:20-40). */
