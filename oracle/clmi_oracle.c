@@ -22,9 +22,12 @@
  * Verlet (orc_pv_evolve) is pinned by its exact free-wave recurrence (momenta differ from the
  * velocity Verlet's by 1/(1 - w^2 dt^2/4)), time reversibility of PV4/PV6, the dt^2 / dt^4 global
  * error orders of PV2 / PV4 and the O(dt^2) Friedmann constraint with expansion.
- * Parity unpinned against the reference (TempLat could not be built or imported): the Gauss-ratio
- * denominator of orc_gauss_u1, which follows definitions/gausslaws.h:43-47 as written.  The
- * spectrum prefactors and the GRF random stream live in oracle.py (same status, stated there).
+ * The Gauss check (orc_gauss_u1, all three outputs: the ratio with its mean-of-sum-of-squares
+ * denominator, <|LHS|> and <|RHS|>) is pinned by checkerboard data on which every lattice average
+ * is exact, with and without charge, and by uncharged plane waves; a forward difference, a wrong
+ * sign of the charge term or any of the plausible wrong denominators fails one of them.
+ * The spectrum prefactors and the GRF random stream live in oracle.py (their status is stated
+ * there).
  */
 #include <math.h>
 #include <stdint.h>

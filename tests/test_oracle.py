@@ -469,6 +469,94 @@ def test_u1_gauss_law_conserved(oracle_mod):
     assert diff < 1e-12
 
 
+def _gauss_lattice(oracle_mod, N, dx, gQ, piA, phi, pi):
+    spec = _u1_spec(gQ=gQ)
+    spec.fStar, spec.omegaStar = 1.7, 0.9  # qnorm = 2/dx (fStar/omegaStar)^2 must enter
+    lat = oracle_mod.OracleLattice(spec, N, dx, 0.01, expansion=False)
+    for d in range(3):
+        lat.upload(PI_U1, 0, d, piA[d])
+        lat.upload(FLD_U1, 0, d, np.zeros((N, N, N)))
+    for c in range(2):
+        lat.upload(FLD_CS, 0, c, phi[c])
+        lat.upload(PI_CS, 0, c, pi[c])
+    qnorm = 2.0 / dx * (spec.fStar / spec.omegaStar) ** 2
+    return lat, qnorm
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_gauss_ratio_checkerboard_closed_form(oracle_mod, charged):
+    """GaussLaws::checkU1 (definitions/gausslaws.h:36-50) on checkerboard data, where every lattice
+    average is exact: pi_A1 = A s, pi_A2 = B s, pi_A3 = C t with s = (-1)^(x+y), t = (-1)^z, and a
+    constant phi = (v, 0) with pi_phi = (0, p s).  The backward differences give LHS_1 = 2 A s / dx,
+    LHS_2 = 2 B s / dx, LHS_3 = 2 C t / dx; the charge density is rho = qnorm gQ Im[conj(pi) phi] =
+    -qnorm gQ p v s, so RHS = -dx rho = R0 s with R0 = dx qnorm gQ p v.  Since s and t are
+    independent fair signs, <|a s + b t|> = max(|a|, |b|), and LHS_1^2 + LHS_2^2 + LHS_3^2 + RHS^2 is
+    the constant 4 (A^2 + B^2 + C^2) / dx^2 + R0^2.  A forward difference, a wrong sign of RHS, a
+    dropped dx or qnorm, or any of the denominators <|LHS| + |RHS|>, <sqrt(sum)>, <LHS^2 + RHS^2>
+    fails at least one of the three outputs."""
+    N, dx, gQ = 8, 0.7, 1.9
+    A, B, C = 0.3, 0.45, 0.05
+    v, p = 1.3, (0.8 if charged else 0.0)
+    x = np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    s = (-1.0) ** (X + Y)
+    t = (-1.0) ** Z
+    piA = [A * s, B * s, C * t]
+    phi = [v * np.ones((N, N, N)), np.zeros((N, N, N))]
+    pi = [np.zeros((N, N, N)), p * s]
+    lat, qnorm = _gauss_lattice(oracle_mod, N, dx, gQ, piA, phi, pi)
+    R0 = dx * qnorm * gQ * p * v
+    a_lhs, b_lhs = 2 * (A + B) / dx, 2 * C / dx
+    want_diff = max(abs(a_lhs - R0), abs(b_lhs))
+    want_sum = 4 * (A * A + B * B + C * C) / dx**2 + R0 * R0
+    got = lat.gauss_u1()
+    assert got[0] == pytest.approx(want_diff / want_sum, rel=1e-13)
+    assert got[1] == pytest.approx(max(abs(a_lhs), abs(b_lhs)), rel=1e-13)
+    assert got[2] == pytest.approx(abs(R0), abs=1e-15, rel=1e-13)
+    # the wrong candidates named above really differ at these numbers
+    wrong = [
+        want_diff / (max(abs(a_lhs), abs(b_lhs)) + abs(R0)),  # <|LHS| + |RHS|>
+        want_diff / math.sqrt(want_sum),  # <sqrt(sum)>
+        want_diff / (4 * ((A + B) ** 2 + C * C) / dx**2 + R0 * R0),  # (sum_i LHS_i)^2 + RHS^2
+    ]
+    if charged:  # with RHS = 0 these two coincide with the right answer
+        wrong += [max(abs(a_lhs + R0), abs(b_lhs)) / want_sum,  # RHS sign
+                  max(abs(-a_lhs - R0), abs(b_lhs)) / want_sum]  # forward differences
+    for w in wrong:
+        assert abs(w - got[0]) > 1e-3 * got[0]
+
+
+def test_gauss_ratio_plane_wave_closed_form(oracle_mod):
+    """Uncharged plane waves pi_A,i = A_i cos(theta + d_i), theta = 2 pi n.x / N, n = (1, 2, 3):
+    LHS_i = (2 A_i / dx) sin(pi n_i / N) cos(theta + d_i - pi n_i / N) exactly, so
+    <LHS_1^2 + LHS_2^2 + LHS_3^2> = sum_i (2 A_i sin(pi n_i / N) / dx)^2 / 2 (2n.x/N is not a multiple
+    of N, so <cos^2> = 1/2 on the lattice).  LHS = |Z| cos(theta + arg Z) with
+    Z = sum_i (2 A_i / dx) sin(pi n_i / N) e^{i (d_i - pi n_i / N)}, and n.x mod N takes every
+    residue equally often, so <|LHS|> = |Z| (1/N) sum_m |cos(2 pi m / N + arg Z)|."""
+    N, dx, gQ = 12, 0.55, 1.1
+    n = np.array([1, 2, 3])
+    amps = np.array([0.4, -0.25, 0.3])
+    dph = np.array([0.3, 1.7, -0.9])
+    x = np.arange(N)
+    X, Y, Z3 = np.meshgrid(x, x, x, indexing="ij")
+    theta = 2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z3) / N
+    piA = [amps[i] * np.cos(theta + dph[i]) for i in range(3)]
+    zero = np.zeros((N, N, N))
+    lat, _ = _gauss_lattice(oracle_mod, N, dx, gQ, piA, [zero + 0.9, zero], [zero, zero])
+    c = 2 * amps / dx * np.sin(np.pi * n / N)
+    want_sum = 0.5 * float(np.sum(c * c))
+    Zc = complex(np.sum(c * np.exp(1j * (dph - np.pi * n / N))))
+    m = np.arange(N)
+    mean_abs_cos = float(np.mean(np.abs(np.cos(2 * np.pi * m / N + np.angle(Zc)))))
+    want_lhs = abs(Zc) * mean_abs_cos
+    got = lat.gauss_u1()
+    assert got[1] == pytest.approx(want_lhs, rel=1e-12)
+    assert got[2] == 0.0
+    assert got[0] == pytest.approx(want_lhs / want_sum, rel=1e-12)
+    # (sum_i LHS_i)^2 in place of sum_i LHS_i^2 would give |Z|^2 / 2 instead
+    assert abs(abs(Zc) ** 2 / 2 - want_sum) > 0.05 * want_sum
+
+
 def _u1_energy(lat):
     av = lat.averages()
     w2 = (lat.spec.omegaStar / lat.spec.fStar) ** 2
