@@ -218,6 +218,9 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
     const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    const int nrows = c->rowk ? c->rplan.rows : c->tiled ? c->tplan.rows : c->blocks_s;
+    if (red && (size_t)r + nrows > c->partial_rows)
+      return fail(c, CLMI_E_INVALID, "partial-sum buffer too small for the scalar pass");
     if (c->rowk) {
       HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream));
       if (red) r += c->rplan.rows;
@@ -266,6 +269,8 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     a.st = c->dstate;
     a.partials = c->partials + (size_t)r * kNumRed;
     a.poly = c->cpoly;
+    if (red && (size_t)r + c->blocks_g > c->partial_rows)
+      return fail(c, CLMI_E_INVALID, "partial-sum buffer too small for the charged-scalar pass");
     if (c->gtiled)
       HIPCHK(c, launch_gauge_tiled(c->f32, c->gTY, drift, kick, red, a, c->blocks_g, c->gXC, c->stream));
     else
@@ -494,9 +499,14 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
       c->blocks_g = gblocks;
     }
   }
-  size_t rows = 2 * (size_t)kMaxPassBlocks + 16;
-  if (c->tiled && (size_t)c->tplan.rows + c->blocks_g + 16 > rows) rows = (size_t)c->tplan.rows + c->blocks_g + 16;
-  if (c->rowk && (size_t)c->rplan.rows + 16 > rows) rows = (size_t)c->rplan.rows + 16;
+  // Partial-sum rows: a reducing step pass writes the scalar sector's rows, then the charged-scalar + U(1)
+  // sector's after them; the measurement kernels write at most kMaxPassBlocks rows each.
+  const size_t srows = g.n_scalars == 0 ? 0
+                       : c->rowk        ? (size_t)c->rplan.rows
+                       : c->tiled       ? (size_t)c->tplan.rows
+                                        : (size_t)c->blocks_s;
+  const size_t grows = g.n_cscalars > 0 ? (size_t)c->blocks_g : 0;
+  const size_t rows = std::max(2 * (size_t)kMaxPassBlocks, srows + grows) + 16;
   c->partial_rows = rows;
   if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||

@@ -154,20 +154,80 @@ def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
         assert res[0][1][k] == pytest.approx(res[1][1][k], rel=1e-13)
 
 
-@pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (64, CLMI_F32_BULK), (128, CLMI_F32_BULK), (256, CLMI_F64)])
-def test_verlet_passes_bit_identical_to_generic(gpu, N, prec):
+def runtime_poly_spec():
+    """Two fields with a potential that is not lphi4's (a chi^4 and a phi^2 term added), so the passes
+    evaluate it from the runtime exponent table rather than the compile-time lphi4 form."""
+    base = bench_spec()
+    return ModelSpec(name="lphi4+", n_scalars=2, alpha=base.alpha, fStar=base.fStar, omegaStar=base.omegaStar,
+                     terms=[(0.25, [4, 0]), (50.0, [2, 2]), (0.1, [0, 4]), (0.3, [2, 0])],
+                     fld0=list(base.fld0), pi0=list(base.pi0))
+
+
+@pytest.mark.parametrize("N,prec", [(64, CLMI_F64), (128, CLMI_F32_BULK), (192, CLMI_F64), (256, CLMI_F32_BULK),
+                                    (256, CLMI_F64), (320, CLMI_F32_BULK), (512, CLMI_F32_BULK)])
+def test_fast_pass_runtime_potential_bit_identical_to_generic(gpu, N, prec):
+    """With a potential outside the compile-time lphi4 form, the full-row pass (64, 128, 192, 256 FP32-bulk,
+    512) and the tiled pass (256 FP64, 320) evaluate the runtime exponent table and still give fields and
+    momenta bit-identical to the generic pass after 40 leapfrog steps on a static background."""
+    from cosmolattice_amd.clmi import CLMI_FLAG_NO_TILED
+
+    say(f"case fast_vs_generic runtime potential N={N} prec={prec}")
+    spec = runtime_poly_spec()
+    ics = scalar_ics(spec, N, np.random.default_rng(N + 7 * prec))
+    res = []
+    for flags in (0, CLMI_FLAG_NO_TILED):
+        lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.set_background(aI=1.1, aSI=1.05)
+        lat.lf_init()
+        lat.lf_evolve(25)
+        lat.lf_sync()
+        lat.lf_evolve(15)
+        res.append([lat.download(k, i, 0) for (k, i) in sorted(ics)])
+        lat.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
+def test_lf_runtime_potential_fp64_parity_64(gpu, oracle_mod):
+    """The same non-lphi4 potential at 64^3 (full-row pass, runtime exponent table) against the oracle:
+    100 leapfrog steps with expansion and measurements every 25, within 1e-10."""
+    say("case lf_runtime_potential_fp64_64")
+    spec = runtime_poly_spec()
+    N, dx, dt = 64, 0.4909, 0.005
+    ics = scalar_ics(spec, N, np.random.default_rng(640))
+    lat = Lattice(spec, N, dx, dt, CLMI_F64)
+    orc = oracle_mod.OracleLattice(spec, N, dx, dt)
+    load_both(lat, orc, ics)
+    init_background(lat, orc)
+    rg = run_reference_loop(lat, 100, 25)
+    ro = run_reference_loop(orc, 100, 25)
+    for k in [(FLD_S, 0), (FLD_S, 1), (PI_S, 0), (PI_S, 1)]:
+        e = relerr(lat.download(*k, 0), orc.download(*k))
+        say(k, e)
+        assert e < FP64_TOL, k
+    np.testing.assert_allclose(rg[:, 1:], ro[:, 1:], rtol=FP64_TOL)
+    lat.close()
+
+
+@pytest.mark.parametrize("N,prec,mm", [(64, CLMI_F64, 15), (64, CLMI_F32_BULK, 15), (128, CLMI_F32_BULK, 15),
+                                       (256, CLMI_F64, 15), (256, CLMI_F64, 0), (320, CLMI_F32_BULK, 0)])
+def test_verlet_passes_bit_identical_to_generic(gpu, N, prec, mm):
     """Velocity Verlet (kick-only and drift+kick passes) and position Verlet (drift-only and drift+kick
-    passes) through the full-row pass give fields and momenta bit-identical to the generic
-    one-site-per-thread pass on a static background (fourth-order compositions, 6 steps)."""
+    passes) give fields and momenta bit-identical to the generic one-site-per-thread pass on a static
+    background (fourth-order compositions, 6 steps): through the full-row pass (memory mode 15 asks for
+    it) and, without that flag, through the default pass, which is the tiled pass at 256^3 FP64 and
+    320^3."""
     from cosmolattice_amd.clmi import CLMI_FLAG_MEMMODE, CLMI_FLAG_NO_TILED, PV_WEIGHTS, VV_WEIGHTS
 
-    say(f"case verlet_vs_generic N={N} prec={prec}")
+    say(f"case verlet_vs_generic N={N} prec={prec} mm={mm}")
     spec = bench_spec()
     rng = np.random.default_rng(3 * N + prec)
     ics = scalar_ics(spec, N, rng)
     for scheme in ("vv", "pv"):
         res = []
-        for flags in (CLMI_FLAG_MEMMODE(15), CLMI_FLAG_NO_TILED):
+        for flags in (CLMI_FLAG_MEMMODE(mm) if mm else 0, CLMI_FLAG_NO_TILED):
             lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
             for (k, i), a in ics.items():
                 lat.upload(k, i, 0, a)
@@ -617,12 +677,15 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
     repeated call is bit-identical.  The cases cover the streaming pass at the production size (256^3,
     8.4 M modes, 1024 workgroups) and with four and with one LDS
     histogram per workgroup (nbins <= 2048 and 2048 < nbins <= 8192), small N (lines shorter than a
-    wave: up to 5 and 14 lines per 64-mode chunk) and the bin-sorted table used above 8192 bins."""
+    wave: up to 5 and 14 lines per 64-mode chunk) and the bin-sorted table used above 8192 bins.  In
+    FP64 the R2C transform reads the field array itself, so the field must come back unchanged at every
+    size checked, including the ensemble sizes 64^3 and 192^3."""
     say("case spectrum")
     spec = bench_spec()
     cases = ((32, 1.0, (CLMI_F64, CLMI_F32_BULK)), (24, 0.5, (CLMI_F64, CLMI_F32_BULK)), (128, 1.0, (CLMI_F64,)),
              (256, 1.0, (CLMI_F32_BULK, CLMI_F64)),
-             (30, 0.7, (CLMI_F64,)), (8, 0.4, (CLMI_F64,)), (24, 0.01, (CLMI_F64,)), (24, 0.002, (CLMI_F64,)))
+             (30, 0.7, (CLMI_F64,)), (8, 0.4, (CLMI_F64,)), (24, 0.01, (CLMI_F64,)), (24, 0.002, (CLMI_F64,)),
+             (64, 1.0, (CLMI_F64,)), (192, 1.0, (CLMI_F64,)))
     for N, dk, precs in cases:
         rng = np.random.default_rng(N)
         f = 1.0 + smooth_random(N, rng) + 0.1 * rng.normal(size=(N, N, N))
