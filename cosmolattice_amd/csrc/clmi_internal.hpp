@@ -330,7 +330,9 @@ struct BinTable {
 };
 hipError_t build_bin_table(BinTable &t, int N, double dk, int nbins);
 void free_bin_table(BinTable &t);
-hipError_t launch_bin_sums(const BinTable &t, const void *fk /*double2*/, double *out, hipStream_t s);
+// out[b] = sum over the modes of bin b of w |f_k|^2 |n|^npow (npow = 0, 1 or 3; w = 2 for the modes the
+// R2C half-space stands for twice).
+hipError_t launch_bin_sums(const BinTable &t, const void *fk /*double2*/, double *out, hipStream_t s, int npow = 0);
 
 struct GrfArgs {
   int N;

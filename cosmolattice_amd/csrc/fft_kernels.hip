@@ -147,6 +147,21 @@ void free_bin_table(BinTable &t)
 
 namespace {
 
+// |n|^NPOW for a mode with |n|^2 = n2 (NPOW = 0, 1 or 3): the weight PS_version 3 puts inside the bin sum
+// (measurements/powerspectrum.h:111-131).
+template <int NPOW>
+__device__ __forceinline__ double npow_weight(uint32_t n2)
+{
+  if constexpr (NPOW == 0) {
+    return 1.0;
+  } else {
+    const double r = sqrt((double)n2);
+    if constexpr (NPOW == 1) return r;
+    else return r * r * r;
+  }
+}
+
+template <int NPOW>
 __global__ __launch_bounds__(256) void k_bin_sums(const uint32_t *modes, const uint32_t *start, const double2 *fk,
                                                   uint32_t Nh, uint32_t Nz2, double *out)
 {
@@ -158,7 +173,13 @@ __global__ __launch_bounds__(256) void k_bin_sums(const uint32_t *modes, const u
     const uint32_t m = modes[i];
     const uint32_t kz = m % Nh;
     const double2 v = fk[m];
-    const double w = (kz == 0 || 2 * kz == Nz2) ? 1.0 : 2.0;
+    double w = (kz == 0 || 2 * kz == Nz2) ? 1.0 : 2.0;
+    if constexpr (NPOW != 0) {
+      const uint32_t line = m / Nh, kx = line / Nz2, ky = line - kx * Nz2;
+      const int nx = 2 * kx <= Nz2 ? (int)kx : (int)kx - (int)Nz2;
+      const int ny = 2 * ky <= Nz2 ? (int)ky : (int)ky - (int)Nz2;
+      w *= npow_weight<NPOW>((uint32_t)(nx * nx + ny * ny) + kz * kz);
+    }
     s += w * (v.x * v.x + v.y * v.y);
   }
   sh[threadIdx.x] = s;
@@ -183,7 +204,7 @@ __device__ __forceinline__ double dpp_f64(double x)
 // Streaming binning.  Wave w of workgroup g reduces the 64-mode chunks [c0, c0 + cpw) of the
 // half-space array (modes m = (kx N + ky) Nh + kz) into its LDS histogram h; the workgroup then writes
 // the sum of its WPG histograms, in wave order, to rows[g].
-template <int WPG>
+template <int WPG, int NPOW>
 __global__ __launch_bounds__(64 * WPG) void k_bin_stream(const double2 *__restrict__ fk,
                                                          const uint16_t *__restrict__ bin_of, uint32_t N,
                                                          uint32_t Nh, uint32_t M, uint32_t nbins, uint32_t cpw,
@@ -230,7 +251,7 @@ __global__ __launch_bounds__(64 * WPG) void k_bin_stream(const double2 *__restri
         const int ny = ky <= half ? (int)ky : (int)ky - (int)N;
         const uint32_t n2 = valid ? (uint32_t)(nx * nx + ny * ny) + kz * kz : 0u;
         const uint32_t b = bin_of[n2];
-        const double w = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
+        const double w = ((kz == 0 || 2 * kz == N) ? 1.0 : 2.0) * npow_weight<NPOW>(n2);
         double x = valid ? w * (v[u].x * v[u].x + v[u].y * v[u].y) : 0.0;
         // keys (line, bin) do not decrease along the chunk; invalid lanes form its tail
         const uint32_t key = valid ? (dl << 16 | b) : 0xFFFFFFFFu;
@@ -556,24 +577,37 @@ __global__ void k_grf_finish(const double *src, double scale, double add, void *
 
 }  // namespace
 
-hipError_t launch_bin_sums(const BinTable &t, const void *fk, double *out, hipStream_t s)
+namespace {
+template <int NPOW>
+hipError_t bin_sums_pow(const BinTable &t, const void *fk, double *out, hipStream_t s)
 {
   if (t.d_bin) {
     const uint32_t N = (uint32_t)t.N, Nh = N / 2 + 1, M = N * N * Nh;
     const size_t lds = (size_t)t.wpg * t.nbins * sizeof(double);
     if (t.wpg == 4)
-      hipLaunchKernelGGL(k_bin_stream<4>, dim3(t.groups), dim3(256), lds, s, (const double2 *)fk, t.d_bin, N, Nh, M,
-                         (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
+      hipLaunchKernelGGL((k_bin_stream<4, NPOW>), dim3(t.groups), dim3(256), lds, s, (const double2 *)fk, t.d_bin, N, Nh,
+                         M, (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
     else
-      hipLaunchKernelGGL(k_bin_stream<1>, dim3(t.groups), dim3(64), lds, s, (const double2 *)fk, t.d_bin, N, Nh, M,
-                         (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
+      hipLaunchKernelGGL((k_bin_stream<1, NPOW>), dim3(t.groups), dim3(64), lds, s, (const double2 *)fk, t.d_bin, N, Nh,
+                         M, (uint32_t)t.nbins, t.chunks_per_wave, t.d_rows);
     hipLaunchKernelGGL(k_bin_rows, dim3((t.nbins + 63) / 64), dim3(1024), 0, s, t.d_rows, (uint32_t)t.groups,
                        (uint32_t)t.nbins, out);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_bin_sums, dim3(t.nbins), dim3(256), 0, s, t.d_modes, t.d_start, (const double2 *)fk,
+  hipLaunchKernelGGL(k_bin_sums<NPOW>, dim3(t.nbins), dim3(256), 0, s, t.d_modes, t.d_start, (const double2 *)fk,
                      (uint32_t)(t.N / 2 + 1), (uint32_t)t.N, out);
   return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_bin_sums(const BinTable &t, const void *fk, double *out, hipStream_t s, int npow)
+{
+  switch (npow) {
+    case 0: return bin_sums_pow<0>(t, fk, out, s);
+    case 1: return bin_sums_pow<1>(t, fk, out, s);
+    case 3: return bin_sums_pow<3>(t, fk, out, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_grf_modes(const GrfArgs &a, int64_t n2max, hipStream_t s)

@@ -682,6 +682,48 @@ def test_grf_statistics_and_structure(oracle_mod):
     assert pi.mean() == pytest.approx(-0.5, abs=1e-12)
 
 
+def test_ps_type2_of_vacuum_grf_is_the_continuum_vacuum_spectrum(oracle_mod):
+    """PowerSpectrumMeasurer's PS_type 2 normalisation (measurements/powerspectrum.h:85-110), applied by
+    the host's power_spectrum to the bin sums, turns the vacuum initial conditions of
+    initializers/fluctuationsgenerator.h:39-71 into the textbook vacuum spectrum of a free field,
+    Delta_phi(k) = (omegaStar/fStar)^2 k^3 / (4 pi^2 omega_k) in program units: with
+    E|phi_k|^2 = (omegaStar/fStar)^2 (L/dx^2)^3 / (2 omega_k) the prefactor (kIR n dx / N)^3 / (2 pi^2 dk)
+    reduces to k^3 / (2 pi^2) because L = N dx.  Checked per bin below the cut, averaged over 6
+    realisations, against k_b^3 / (4 pi^2) times the bin's mean of 1/omega_k (PS_version 1: k_b from the
+    bin's mean |n|), within 5 standard errors.  A dropped factor 2, a wrong power of N or dx, or a
+    missing omegaStar/fStar fails by far more."""
+    from cosmolattice_amd.clmi import power_spectrum
+
+    N, kIR, m2 = 64, 0.1, 1.3
+    dx = 2 * math.pi / kIR / N
+    spec = lphi4(1.0)
+    spec.fStar, spec.omegaStar = 1.0, 0.8  # (omegaStar/fStar)^2 = 0.64 must appear
+    kcut = 20.5 * kIR
+    nseeds = 6
+    acc = None
+    for seed in range(nseeds):
+        phi, _ = oracle_mod.grf_scalar(N, dx, spec, 900 + seed, 0, m2, 0.0, kIR, kcut, 0.0, 0.0)
+        s, c, mn = oracle_mod.spectrum_bins(phi, 1.0)
+        _, ps = power_spectrum(s, c, mn, N, kIR, 1.0, ps_type=2, ps_version=1)
+        acc = ps if acc is None else acc + ps
+    ps = acc / nseeds
+    # the bin's mean of 1/omega_k over its modes (full mode set, as the bins count them)
+    nn = oracle_mod.mode_norms(N).ravel()
+    b = np.minimum(np.floor(nn + 0.5).astype(np.int64), len(c) - 1)
+    inv_om = np.bincount(b, weights=1.0 / np.sqrt((nn * kIR) ** 2 + m2), minlength=len(c)) / np.maximum(c, 1)
+    w2 = (spec.omegaStar / spec.fStar) ** 2
+    want = w2 * (mn * kIR) ** 3 / (4 * math.pi ** 2) * inv_om
+    checked = 0
+    for k in range(2, len(c)):
+        if (k + 0.5) * kIR > kcut or c[k] < 200:  # whole bins below the cut with enough modes
+            continue
+        # |phi_k|^2 is exponentially distributed; the bin holds about c/2 independent complex modes
+        se = 1.0 / math.sqrt(nseeds * c[k] / 2.0)
+        assert abs(ps[k] / want[k] - 1.0) < 5 * se, (k, ps[k], want[k])
+        checked += 1
+    assert checked >= 10
+
+
 # ------------------------------------------------------------------------------------------------
 # Complex scalar + U(1) initial conditions
 # ------------------------------------------------------------------------------------------------
