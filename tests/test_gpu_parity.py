@@ -276,7 +276,8 @@ def test_autotune_keeps_state_and_results(gpu, prec):
 
 
 def test_lf_lphi4_fp64_parity_64_fast_pass(gpu, oracle_mod):
-    """64^3 (full-row pass with the background update folded into it) against the oracle: 100 leapfrog steps, measurements every 25."""
+    """64^3 (full-row pass, then the separate background-update kernel) against the oracle: 100 leapfrog
+    steps, measurements every 25."""
     say("case lf_lphi4_fp64_64")
     spec = bench_spec()
     N, dx, dt = 64, 0.4909, 0.005
