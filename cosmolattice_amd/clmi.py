@@ -123,6 +123,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_spectrum_nbins": (C.c_int, [P, C.c_double]),
         "clmi_spectrum_bins": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "clmi_spectrum_bins_npow": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "clmi_grf_scalar": (C.c_int, [P, C.c_int, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double]),
         "clmi_grf_u1": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_double,
@@ -424,14 +426,27 @@ class Lattice:
         self._check(self.lib.clmi_monomial_averages(self._h, out), "monomial_averages")
         return list(out)[:len(self.spec.terms)]
 
-    def spectrum_bins(self, kind: int, index: int, comp: int = 0, dk: float = 1.0):
+    def spectrum_bins(self, kind: int, index: int, comp: int = 0, dk: float = 1.0, npow: int = 0):
+        """(sum over the bin's modes of |f_n|^2 |n|^npow, mode count, mean |n|) per bin."""
         nb = self.lib.clmi_spectrum_nbins(self._h, dk)
         s = np.zeros(nb)
         cnt = np.zeros(nb)
         mn = np.zeros(nb)
-        self._check(self.lib.clmi_spectrum_bins(self._h, kind, index, comp, dk, nb, _dptr(s), _dptr(cnt),
-                                                _dptr(mn)), "spectrum_bins")
+        if npow == 0:
+            rc = self.lib.clmi_spectrum_bins(self._h, kind, index, comp, dk, nb, _dptr(s), _dptr(cnt), _dptr(mn))
+        else:
+            rc = self.lib.clmi_spectrum_bins_npow(self._h, kind, index, comp, dk, nb, npow, _dptr(s), _dptr(cnt),
+                                                  _dptr(mn))
+        self._check(rc, "spectrum_bins")
         return s, cnt, mn
+
+    def power_spectrum(self, kind: int, index: int, comp: int = 0, dk: float = 1.0, ps_type: int = 1,
+                       ps_version: int = 1) -> tuple:
+        """PowerSpectrumMeasurer::powerSpectrum of one component (measurements/powerspectrum.h:85-134):
+        PS_version 3 bins |n| f (PS_type 1) or |n|^3 f (PS_type 2) instead of f."""
+        npow = (1 if ps_type == 1 else 3) if ps_version == 3 else 0
+        s, c, m = self.spectrum_bins(kind, index, comp, dk, npow)
+        return power_spectrum(s, c, m, self.N, 2 * math.pi / (self.N * self.dx), dk, ps_type, ps_version)
 
     def grf_scalar(self, index: int, seed: int, stream: int, mass2: float, aDot: float, kIR: float,
                    kCutOff: float, fld0: float, pi0: float):
@@ -492,15 +507,28 @@ class Lattice:
 
 def power_spectrum(sum_f2, count, mean_n, N: int, kIR: float, dk: float = 1.0, ps_type: int = 1,
                    ps_version: int = 1) -> tuple:
-    """PowerSpectrumMeasurer::powerSpectrum normalisation (measurements/powerspectrum.h:85-110) applied
+    """PowerSpectrumMeasurer::powerSpectrum normalisation (measurements/powerspectrum.h:85-134) applied
     to the binned sums: returns (k values = n_b * kIR, spectrum).  PS_version 1 uses each bin's
-    mean |n|, PS_version 2 the bin's nominal centre b * dk."""
+    mean |n|, PS_version 2 the bin's nominal centre b * dk.  PS_version 3 takes the |n|^p-weighted sums
+    (p = 1 for PS_type 1, 3 for PS_type 2, from clmi_spectrum_bins_npow): the |n| factor is then inside
+    the bin sum, and the prefactor has neither the bin's |n| nor 1/dk (powerspectrum.h:111-131)."""
     sum_f2 = np.asarray(sum_f2)
     count = np.asarray(count)
     nb = len(sum_f2)
     nominal = np.arange(nb) * dk
-    nt = np.asarray(mean_n) if ps_version == 1 else nominal
     dx = 2 * math.pi / kIR / N
+    if ps_version == 3:
+        if ps_type == 1:
+            ps = kIR * dx / N ** 5 / 2 / math.pi * sum_f2
+        elif ps_type == 2:
+            mean = np.where(count > 0, sum_f2 / np.maximum(count, 1), 0.0)
+            ps = (kIR * dx / N) ** 3 / 2 / math.pi ** 2 * mean
+        else:
+            raise ValueError("ps_type must be 1 or 2")
+        return nominal * kIR, ps
+    if ps_version not in (1, 2):
+        raise ValueError("ps_version must be 1, 2 or 3")
+    nt = np.asarray(mean_n) if ps_version == 1 else nominal
     if ps_type == 1:
         ps = kIR * nt * dx / dk / N ** 5 / 2 / math.pi * sum_f2
     elif ps_type == 2:

@@ -1013,9 +1013,16 @@ int clmi_spectrum_nbins(const clmi_ctx *c, double dk)
 int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, int nbins, double *sum_f2,
                        double *count, double *mean_n)
 {
+  return clmi_spectrum_bins_npow(c, kind, index, comp, dk, nbins, 0, sum_f2, count, mean_n);
+}
+
+int clmi_spectrum_bins_npow(clmi_ctx *c, int kind, int index, int comp, double dk, int nbins, int npow,
+                            double *sum_f2, double *count, double *mean_n)
+{
   CHK(check_ctx(c));
   if (!(dk > 0) || nbins != clmi_spectrum_nbins(c, dk) || !sum_f2)
     return fail(c, CLMI_E_INVALID, "nbins must equal clmi_spectrum_nbins(ctx, dk)");
+  if (npow != 0 && npow != 1 && npow != 3) return fail(c, CLMI_E_INVALID, "npow must be 0, 1 or 3");
   CHK(flush(c));
   double *fft_in = nullptr;
   if (kind == CLMI_MAG_U1) {
@@ -1049,7 +1056,7 @@ int clmi_spectrum_bins(clmi_ctx *c, int kind, int index, int comp, double dk, in
     HIPCHK(c, hipMalloc(&c->bin_out, sizeof(double) * nbins));
     c->bin_out_n = nbins;
   }
-  HIPCHK(c, launch_bin_sums(c->bins, c->fk, c->bin_out, c->stream));
+  HIPCHK(c, launch_bin_sums(c->bins, c->fk, c->bin_out, c->stream, npow));
   HIPCHK(c, hipMemcpyAsync(sum_f2, c->bin_out, sizeof(double) * nbins, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int b = 0; b < nbins; ++b) {

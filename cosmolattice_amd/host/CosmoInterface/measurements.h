@@ -64,7 +64,9 @@ template <class Model> EnergyTotals energyTotals(Model &model, const clmi_averag
 }
 
 // Power spectrum of one real component with the normalisations of PowerSpectrumMeasurer
-// (measurements/powerspectrum.h:85-110): returns (k = nominal bin centre x kIR, PS) per bin.
+// (measurements/powerspectrum.h:85-134): returns (k = nominal bin centre x kIR, PS) per bin.
+// PS_version 3 bins |n| f (PS_type 1) or |n|^3 f (PS_type 2): the |n| weight sits inside the bin sum
+// and the prefactor carries neither the bin's |n| nor 1/dk.
 template <class Model>
 void powerSpectrum(Model &model, int kind, int index, int comp, double dk, int psType, int psVersion,
                    std::vector<double> &k, std::vector<double> &ps)
@@ -73,14 +75,22 @@ void powerSpectrum(Model &model, int kind, int index, int comp, double dk, int p
   const int nb = clmi_spectrum_nbins(ctx, dk);
   if (nb < 1) throw std::runtime_error("bad spectrum bin width");
   std::vector<double> s(nb), cnt(nb), mn(nb);
-  model.check(clmi_spectrum_bins(ctx, kind, index, comp, dk, nb, s.data(), cnt.data(), mn.data()),
-              "clmi_spectrum_bins");
+  const int npow = psVersion == 3 ? (psType == 1 ? 1 : 3) : 0;
+  model.check(clmi_spectrum_bins_npow(ctx, kind, index, comp, dk, nb, npow, s.data(), cnt.data(), mn.data()),
+              "clmi_spectrum_bins_npow");
   const double N = (double)model.N, kIR = model.kIR, dx = model.dx, pi = 3.14159265358979323846;
   k.assign(nb, 0.0);
   ps.assign(nb, 0.0);
   for (int b = 0; b < nb; ++b) {
-    const double nt = psVersion == 1 ? mn[b] : b * dk;
     k[b] = b * dk * kIR;
+    if (psVersion == 3) {
+      if (psType == 1)
+        ps[b] = kIR * dx / std::pow(N, 5) / 2 / pi * s[b];
+      else
+        ps[b] = std::pow(kIR * dx / N, 3) / 2 / (pi * pi) * (cnt[b] > 0 ? s[b] / cnt[b] : 0.0);
+      continue;
+    }
+    const double nt = psVersion == 1 ? mn[b] : b * dk;
     if (psType == 1)
       ps[b] = kIR * nt * dx / dk / std::pow(N, 5) / 2 / pi * s[b];
     else

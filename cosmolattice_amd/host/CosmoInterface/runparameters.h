@@ -94,8 +94,9 @@ public:
     if (withGWs) throw RunParametersInconsistent("gravitational waves are not part of this library's lattice path.");
     if (powerSpectrumType < 1 || powerSpectrumType > 2)
       throw RunParametersInconsistent("PS_type must be 1 or 2 for the binned spectra of this library.");
-    if (powerSpectrumVersion < 1 || powerSpectrumVersion > 2)
-      throw RunParametersInconsistent("PS_version must be 1 or 2 for the binned spectra of this library.");
+    if (powerSpectrumVersion < 1 || powerSpectrumVersion > 3)
+      throw RunParametersInconsistent("powerSpectrumVersion " + std::to_string(powerSpectrumVersion) +
+                                      " is not a valid powerSpectrumVersion.");
   }
 
   LatticeParameters<T> getLatParams() const { return LatticeParameters<T>(dx, lSide, kIR, N); }

@@ -223,6 +223,11 @@ int clmi_spectrum_nbins(const clmi_ctx *ctx, double dk);
  * the modes in the bin (0 for an empty bin). */
 int clmi_spectrum_bins(clmi_ctx *ctx, int kind, int index, int comp, double dk, int nbins, double *sum_f2,
                        double *count, double *mean_n);
+/* clmi_spectrum_bins with every mode's |f_n|^2 weighted by |n|^npow inside the bin sum (npow = 0, 1
+ * or 3): the projections PS_version 3 measures, |n| f for PS_type 1 and |n|^3 f for PS_type 2
+ * (measurements/powerspectrum.h:111-131).  npow = 0 is clmi_spectrum_bins. */
+int clmi_spectrum_bins_npow(clmi_ctx *ctx, int kind, int index, int comp, double dk, int nbins, int npow,
+                            double *sum_f2, double *count, double *mean_n);
 
 /* Gaussian random field initial conditions for scalar singlet `index` on the device
  * (initializers/fluctuationsgenerator.h:39-71,103-150, scalarsingletinitializer.h:41-77,283-284):

@@ -283,14 +283,18 @@ def magnetic_field(A, dx: float):
     return [(fd(A[(i + 2) % 3], (i + 1) % 3) - fd(A[(i + 1) % 3], (i + 2) % 3)) / dx for i in range(3)]
 
 
-def spectrum_bins(f: np.ndarray, dk: float = 1.0):
+def spectrum_bins(f: np.ndarray, dk: float = 1.0, npow: int = 0):
     """|f_n|^2 summed over |n|-shells of the unnormalised forward DFT (full N^3 mode set); returns
-    (sum |f_n|^2, number of modes, mean |n|) per bin, bin b = min(floor(|n|/dk + 1/2), nbins-1)."""
+    (sum |f_n|^2, number of modes, mean |n|) per bin, bin b = min(floor(|n|/dk + 1/2), nbins-1).
+    npow > 0 weights each mode by |n|^npow inside the sum: the projections of |n| f and |n|^3 f that
+    PS_version 3 bins (measurements/powerspectrum.h:111-131)."""
     N = f.shape[0]
     nb = spectrum_nbins(N, dk)
     fk = np.fft.fftn(f)
     p = (fk.real ** 2 + fk.imag ** 2).ravel()
     nn = mode_norms(N).ravel()
+    if npow:
+        p = p * nn ** npow
     b = np.minimum(np.floor(nn / dk + 0.5).astype(np.int64), nb - 1)
     s = np.bincount(b, weights=p, minlength=nb)
     cnt = np.bincount(b, minlength=nb).astype(np.float64)

@@ -310,6 +310,45 @@ def test_host_spectrum_normalisation_parseval(gpu, tmp_path, version):
             assert total == pytest.approx(rms ** 2, rel=1e-7), (col, total, rms ** 2)
 
 
+def _spectrum_blocks(path):
+    text = path.read_text()
+    return [np.loadtxt(b.splitlines()) for b in
+            ["\n".join(ln for ln in blk.splitlines() if not ln.startswith("#")) for blk in text.split("\n\n")]
+            if b.strip()]
+
+
+@pytest.mark.parametrize("ps_type", [1, 2])
+def test_host_spectrum_version3_against_version1(gpu, tmp_path, ps_type):
+    """PS_version 3 (measurements/powerspectrum.h:111-131) puts |n| (PS_type 1) or |n|^3 (PS_type 2)
+    inside the bin sum and drops the bin's |n| and 1/dk from the prefactor.  Two runs of the lphi4
+    program from the same seed, PS_version 1 and 3 with bin width dk = 0.5, see the same fields, so in
+    bin b (|n| in [(b - 1/2) dk, (b + 1/2) dk)) the ratio PS3 / PS1 lies within
+    dk [(b - 1/2)/(b + 1/2), (b + 1/2)/(b - 1/2)]^p: a kept 1/dk doubles it, a missing |n| weight
+    divides it by about b dk."""
+    say(f"case host spectrum v3 type {ps_type}")
+    N, dx, dt, steps, every, dk = 32, 0.6, 0.005, 4, 2, 0.5
+    out = {}
+    for version in (1, 3):
+        d = tmp_path / f"v{version}"
+        d.mkdir()
+        extra = f"PS_type = {ps_type}\nPS_version = {version}\nbaseSeed = 5\nkCutOff = 2.0\ndeltaKBin = {dk}\n"
+        run_program("lphi4", d, lphi4_params(N, dx, dt, steps, every, extra=extra))
+        out[version] = _spectrum_blocks(d / "spectra_scalar_1.txt")
+    p = 1 if ps_type == 1 else 3
+    assert len(out[1]) == len(out[3]) >= 2
+    for b1, b3 in zip(out[1], out[3]):
+        np.testing.assert_array_equal(b1[:, 0], b3[:, 0])
+        for col in (1, 2):
+            b = np.arange(b1.shape[0])
+            # the last bin also takes the corner modes past its upper edge
+            ok = (b >= 2) & (b < b1.shape[0] - 1) & (b1[:, col] > 1e-30 * b1[:, col].max())
+            ratio = b3[ok, col] / b1[ok, col]
+            lo = dk * ((b[ok] - 0.5) / (b[ok] + 0.5)) ** p
+            hi = dk * ((b[ok] + 0.5) / (b[ok] - 0.5)) ** p
+            assert ok.sum() > 10
+            assert np.all(ratio >= lo * (1 - 1e-12)) and np.all(ratio <= hi * (1 + 1e-12)), (col, ratio, lo, hi)
+
+
 def test_host_backup_and_restart(gpu, tmp_path):
     """A run restarted from its own backup (load_file, as SimulationManager::loadSim) ends where the
     uninterrupted run ends, to rounding: the restart recomputes the cached averages

@@ -707,6 +707,38 @@ def test_spectrum_bins_parity(gpu, oracle_mod):
             lat.close()
 
 
+@pytest.mark.parametrize("N,dk", [(32, 1.0), (256, 1.0), (24, 0.002)])
+def test_spectrum_version3_weighted_bins(gpu, oracle_mod, N, dk):
+    """PS_version 3 bins |n|^p |f_n|^2 (p = 1 for PS_type 1, 3 for PS_type 2;
+    measurements/powerspectrum.h:111-131): the weighted sums of both binning passes (streaming at
+    dk = 1, the bin-sorted table at dk = 0.002) equal the numpy restatement within 1e-12, and a plane
+    wave cos(2 pi n0.x / N) of amplitude A gives the closed forms PS_1 = |n0| A^2 / 2 and
+    PS_2 = 2 pi |n0|^3 A^2 / count in the bin of |n0| (the host's prefactors, through the C ABI)."""
+    say("case spectrum version 3", N, dk)
+    spec = bench_spec()
+    rng = np.random.default_rng(N + 3)
+    f = 1.0 + smooth_random(N, rng) + 0.1 * rng.normal(size=(N, N, N))
+    lat = Lattice(spec, N, 0.5, 0.01, CLMI_F64)
+    lat.upload(FLD_S, 1, 0, f)
+    for p in (1, 3):
+        s, c, m = lat.spectrum_bins(FLD_S, 1, 0, dk, npow=p)
+        so, co, mo = oracle_mod.spectrum_bins(f, dk, npow=p)
+        np.testing.assert_array_equal(c, co)
+        np.testing.assert_allclose(s, so, rtol=1e-12, atol=1e-12 * so.max())
+    if dk == 1.0:
+        n0, A = (3, 4, 0), 0.7  # |n0| = 5
+        x = np.arange(N)
+        X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+        lat.upload(FLD_S, 1, 0, A * np.cos(2 * np.pi * (n0[0] * X + n0[1] * Y + n0[2] * Z) / N))
+        _, ps1 = lat.power_spectrum(FLD_S, 1, 0, 1.0, ps_type=1, ps_version=3)
+        _, ps2 = lat.power_spectrum(FLD_S, 1, 0, 1.0, ps_type=2, ps_version=3)
+        _, c, _ = lat.spectrum_bins(FLD_S, 1, 0, 1.0)
+        assert ps1[5] == pytest.approx(5 * A * A / 2, rel=1e-12)
+        assert ps2[5] == pytest.approx(2 * np.pi * 125 * A * A / c[5], rel=1e-12)
+        assert np.abs(np.delete(ps1, 5)).max() < 1e-12 * ps1[5]
+    lat.close()
+
+
 def test_spectrum_leaves_fp64_field_unchanged_512(gpu):
     """In FP64 the spectrum's R2C transform reads the field array itself (out of place): at 512^3, where
     hipFFT plans differently from the smaller sizes, the field comes back bit-identical and a repeated
