@@ -7,9 +7,11 @@ units, fStar/MPl = 2.34 so the expansion is live), kIR = 0.05, dt = 0.005, seede
 
 Parametric resonance amplifies any perturbation of the initial state exponentially, so two runs that
 differ at rounding level end at t = 20 with visibly different spectra.  The floor is measured rather
-than assumed: the FP64 mode run again from the same initial state with its momenta rounded to FP32
-(exactly what FP32-bulk storage does to the initial state: fields are split hi/lo without loss, momenta
-are FP32).  FP32-bulk must then be as close to FP64 as that floor allows.
+than assumed, with two FP64 runs from perturbed copies of the same initial state: one with its momenta
+rounded to FP32 (what FP32-bulk storage does to the initial state: fields are split hi/lo without loss,
+momenta are FP32), one with every field and momentum value multiplied by (1 + 2^-24 u), u uniform in
+[-1, 1] (an FP32 rounding-level perturbation of everything).  FP32-bulk rounds its momenta at every
+step, not once, so its error is compared with a stated multiple of the larger floor.
 """
 import math
 
@@ -31,7 +33,7 @@ def band_errors(pa, pb, nband=8):
     return np.array([abs(pa[lo:hi].sum() - pb[lo:hi].sum()) / pb[lo:hi].sum() for lo, hi in zip(edges[:-1], edges[1:])])
 
 
-def run(prec, round_pi_f32=False):
+def run(prec, ic=None):
     spec = lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
     dx = 2 * math.pi / KIR / N
     lat = Lattice(spec, N, dx, DT, prec, expansion=True)
@@ -41,8 +43,13 @@ def run(prec, round_pi_f32=False):
     m2 = spec.mass2(spec.fld0)
     for f in range(2):
         lat.grf_scalar(f, SEED, f, m2[f], aDot0, KIR, N * KIR / 4, spec.fld0[f], spec.pi0[f])
-        if round_pi_f32:
+        if ic == "pi_f32":
             lat.upload(PI_S, f, 0, lat.download(PI_S, f, 0).astype(np.float32).astype(np.float64))
+        elif ic == "rel_2^-24":
+            rng = np.random.default_rng(SEED + 17 + f)
+            for k in (FLD_S, PI_S):
+                a = lat.download(k, f, 0)
+                lat.upload(k, f, 0, a * (1.0 + 2.0 ** -24 * rng.uniform(-1.0, 1.0, size=a.shape)))
     _, _, rhs = lat.hubble_constraint()
     lat.set_background(aDotI=math.sqrt(rhs))
     lat.lf_init()
@@ -61,21 +68,28 @@ def run(prec, round_pi_f32=False):
 
 
 def test_f32_bulk_at_full_horizon_within_chaotic_floor(gpu):
-    """After 4000 steps: the Friedmann violation of both modes stays below 1e-4 over the whole run; the
-    band-averaged chi and phi spectrum errors of FP32-bulk against FP64 are below 1e-3, or below 3x the
-    error of the FP64 run whose initial momenta were rounded to FP32 (the chaotic floor), whichever is
-    larger; and the scale factors agree to 1e-6."""
+    """After 4000 steps: the Friedmann violation of FP32-bulk equals FP64's within 1e-7 at every check
+    (the precision mode adds nothing visible to the leapfrog's own O(dt^2) violation, whose maximum at
+    dt = 0.005 is about 1.6e-4 for this seed and is bounded here by 3e-4); the band-averaged chi and phi
+    spectrum errors of FP32-bulk against FP64 are below 3x the larger of the two chaotic floors; and the
+    scale factors agree to 1e-6.  Every figure is printed before the first assertion."""
     say("case accuracy at t = 20")
     f64, s64, a64 = run(CLMI_F64)
     f32, s32, a32 = run(CLMI_F32_BULK)
-    fl, sfl, afl = run(CLMI_F64, round_pi_f32=True)
-    for name, f in (("f64", f64), ("f32_bulk", f32), ("floor", fl)):
-        say(name, "max |Friedmann|", np.max(np.abs(f)))
-    assert np.max(np.abs(f64)) < 1e-4 and np.max(np.abs(f32)) < 1e-4
+    fl1, sfl1, afl1 = run(CLMI_F64, ic="pi_f32")
+    fl2, sfl2, afl2 = run(CLMI_F64, ic="rel_2^-24")
+    for name, f in (("f64", f64), ("f32_bulk", f32), ("floor pi_f32", fl1), ("floor rel_2^-24", fl2)):
+        say(name, "max |Friedmann|", np.max(np.abs(f)), "end", f[-1])
+    say("max |Friedmann f32_bulk - f64|", np.max(np.abs(f32 - f64)))
+    ratios = {}
     for fi, name in ((1, "chi"), (0, "phi")):
-        e32 = band_errors(s32[fi], s64[fi])
-        efl = band_errors(sfl[fi], s64[fi])
-        say(name, "band error f32_bulk mean", e32.mean(), "max", e32.max(), "| floor mean", efl.mean(), "max", efl.max())
-        assert e32.mean() < max(1e-3, 3 * efl.mean()), name
-    say("aI", a64, a32, afl)
+        e32 = band_errors(s32[fi], s64[fi]).mean()
+        efl = max(band_errors(sfl1[fi], s64[fi]).mean(), band_errors(sfl2[fi], s64[fi]).mean())
+        ratios[name] = e32 / efl
+        say(name, "band error f32_bulk mean", e32, "| larger floor mean", efl, "| ratio", e32 / efl)
+    say("aI", a64, a32, afl1, afl2)
+    assert np.max(np.abs(f64)) < 3e-4 and np.max(np.abs(f32)) < 3e-4
+    assert np.max(np.abs(f32 - f64)) < 1e-7
+    for name, r in ratios.items():
+        assert r < 3.0, name
     assert abs(a32 - a64) / a64 < 1e-6
