@@ -35,6 +35,7 @@ HIP_SOURCES = [
     "fft_kernels.hip",
     "adjoint_kernels.hip",
     "clmi_api.hip",
+    "noalloc.hip",
 ]
 
 HIPCC_FLAGS = [

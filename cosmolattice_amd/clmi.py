@@ -142,6 +142,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
         "clmi_autotune": (C.c_int, [P, C.c_int, C.c_char_p, C.c_size_t]),
+        "clmi_field_stats_noalloc": (C.c_int, [P, C.c_int, C.c_double, P, P]),
+        "clmi_noalloc_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -153,6 +155,18 @@ def load_library(path: Path | None = None) -> C.CDLL:
 
 class ClmiError(RuntimeError):
     pass
+
+
+def field_stats_noalloc(d_field: int, N: int, dx: float, d_out: int, stream: int = 0) -> None:
+    """clmi_field_stats_noalloc: {<f>, <f^2>, <sum_i ((f(x+i) - f(x))/dx)^2>} of a caller-owned FP64
+    device array of N^3 doubles into 3 caller-owned device doubles, enqueued on `stream` (device
+    addresses and the stream handle as integers, e.g. a torch tensor's data_ptr() and a torch stream's
+    cuda_stream).  Allocates nothing; the caller synchronises the stream before reading d_out."""
+    lib = load_library()
+    rc = lib.clmi_field_stats_noalloc(C.c_void_p(d_field or None), N, dx, C.c_void_p(d_out or None),
+                                      C.c_void_p(stream or None))
+    if rc != CLMI_OK:
+        raise ClmiError(f"field_stats_noalloc: {lib.clmi_noalloc_last_error().decode()} (code {rc})")
 
 
 def _dptr(a: np.ndarray):

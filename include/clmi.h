@@ -314,6 +314,22 @@ int clmi_profile_read(clmi_ctx *ctx, double *total_ms, int64_t *launches, double
  * can be scaled against a kernel whose traffic is known.  Allocates and frees its two buffers. */
 int clmi_copy_calibration(clmi_ctx *ctx, size_t bytes, int iters);
 
+/* ---- allocation-free variant ------------------------------------------------------------------ */
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): clmi_field_stats on a
+ * caller-owned FP64 array d_field of N*N*N doubles (dense [x][y][z], z fastest, periodic, on the
+ * current device), with the output d_out = {<f>, <f^2>, <sum_i ((f(x+i) - f(x))/dx)^2>} written to
+ * caller-owned device memory (3 doubles), ordered on `stream` (a hipStream_t; NULL = the null stream).
+ * The call returns once the work is enqueued.  Same kernels and summation order as clmi_field_stats,
+ * so the results are bit-identical to it.  The partial sums go to a fixed 262 208-byte static device
+ * buffer that each GPU holds until the process ends; the call allocates nothing.  Calls that overlap
+ * on one GPU (other streams or host threads) share that buffer and race.  N must lie in [2, 1024];
+ * otherwise, or for a null pointer, it returns CLMI_E_INVALID before launching anything, and
+ * clmi_noalloc_last_error() holds the message. */
+int clmi_field_stats_noalloc(const double *d_field, int N, double dx, double *d_out, void *stream);
+const char *clmi_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
