@@ -64,12 +64,14 @@ template <class Model> EnergyTotals energyTotals(Model &model, const clmi_averag
 }
 
 // Power spectrum of one real component with the normalisations of PowerSpectrumMeasurer
-// (measurements/powerspectrum.h:85-134): returns (k = nominal bin centre x kIR, PS) per bin.
+// (measurements/powerspectrum.h:85-134): returns (k = nominal bin centre x kIR, PS) per bin, and, when
+// asked, the mean |k| of the bin's modes (kavg) and the bin's mode count (mult).
 // PS_version 3 bins |n| f (PS_type 1) or |n|^3 f (PS_type 2): the |n| weight sits inside the bin sum
 // and the prefactor carries neither the bin's |n| nor 1/dk.
 template <class Model>
 void powerSpectrum(Model &model, int kind, int index, int comp, double dk, int psType, int psVersion,
-                   std::vector<double> &k, std::vector<double> &ps)
+                   std::vector<double> &k, std::vector<double> &ps, std::vector<double> *kavg = nullptr,
+                   std::vector<double> *mult = nullptr)
 {
   clmi_ctx *ctx = model.context();
   const int nb = clmi_spectrum_nbins(ctx, dk);
@@ -81,6 +83,11 @@ void powerSpectrum(Model &model, int kind, int index, int comp, double dk, int p
   const double N = (double)model.N, kIR = model.kIR, dx = model.dx, pi = 3.14159265358979323846;
   k.assign(nb, 0.0);
   ps.assign(nb, 0.0);
+  if (kavg) {
+    kavg->assign(nb, 0.0);
+    for (int b = 0; b < nb; ++b) (*kavg)[b] = mn[b] * kIR;
+  }
+  if (mult) *mult = cnt;
   for (int b = 0; b < nb; ++b) {
     k[b] = b * dk * kIR;
     if (psVersion == 3) {
@@ -108,7 +115,7 @@ public:
         infreqOutputFreq((int)std::lround(par.tOutInfreq / par.dt)),
         rareOutputFreq((int)std::lround(par.tOutRareFreq / par.dt)), expansion(par.expansion),
         printHeaders(par.printHeaders), dk(par.deltaKBin), psType(par.powerSpectrumType),
-        psVersion(par.powerSpectrumVersion), dir(par.outFn),
+        psVersion(par.powerSpectrumVersion), verbosity(par.spectraVerbosity), dir(par.outFn),
         energyCons(par.outFn + "average_energy_conservation.txt",
                    expansion ? std::vector<std::string>{"t", "rel_diff_friedmann", "LHS_friedmann", "RHS_friedmann"}
                              : std::vector<std::string>{"t", "energy_cons"},
@@ -279,10 +286,14 @@ private:
     }
   }
 
-  // One block of a spectra file (spectrumsaverstd.h:54-96): a row per bin with k and the columns,
-  // then an empty line.  The header line is written once, before the first block.
-  void saveSpectra(const std::string &name, T t, const std::vector<double> &k,
-                   const std::vector<std::vector<double>> &cols, const std::vector<std::string> &names)
+  // One block of a spectra file (spectrumsaverstd.h:54-96): a row per bin, then an empty line; the
+  // header line is written once, before the first block.  The bin columns follow spectraVerbosity
+  // (spectrumsaverstd.h:56-82): 0 -> the bin's central k; 1 -> the mean |k| of the bin's modes (the
+  // HDF5 writer's momBinAverage, hdf5/spectrumsaverhdf5.h:98-100); 2 -> central k, mean |k| and the
+  // bin's mode count (momMultiplicity).  Then one column per spectrum.
+  void saveSpectra(const std::string &name, T t, const std::vector<double> &k, const std::vector<double> &kavg,
+                   const std::vector<double> &mult, const std::vector<std::vector<double>> &cols,
+                   const std::vector<std::string> &names)
   {
     const std::string fn = dir + "spectra_" + name + ".txt";
     const bool started = spectraStarted_.count(fn) > 0;
@@ -290,12 +301,16 @@ private:
     if (!out) throw std::runtime_error("cannot open " + fn);
     out << std::setprecision(15);
     if (!started && printHeaders) {
-      out << "#k";
+      if (verbosity == 0) out << "#k";
+      else if (verbosity == 1) out << "#k_avg";
+      else out << "#k    k_avg    multiplicity";
       for (const auto &n : names) out << "    " << n;
       out << "    t = " << t << "\n";
     }
     for (size_t b = 0; b < k.size(); ++b) {
-      out << k[b];
+      if (verbosity == 0) out << k[b];
+      else if (verbosity == 1) out << kavg[b];
+      else out << k[b] << "    " << kavg[b] << "    " << mult[b];
       for (const auto &c : cols) out << "    " << c[b];
       out << "\n";
     }
@@ -310,11 +325,11 @@ private:
   void measureSpectra(T t, Model &model)
   {
     const double a = model.aI, al = model.alpha;
-    std::vector<double> k, pf, pp;
+    std::vector<double> k, kavg, mult, pf, pp;
     auto sum = [&](int kind, int idx, int ncomp, double scale) {
       std::vector<double> acc, c;
       for (int q = 0; q < ncomp; ++q) {
-        powerSpectrum(model, kind, idx, q, dk, psType, psVersion, k, c);
+        powerSpectrum(model, kind, idx, q, dk, psType, psVersion, k, c, &kavg, &mult);
         if (acc.empty()) acc.assign(c.size(), 0.0);
         for (size_t b = 0; b < c.size(); ++b) acc[b] += c[b];
       }
@@ -324,23 +339,24 @@ private:
     for (size_t i = 0; i < Model::Ns; ++i) {
       pf = sum(CLMI_FLD_S, (int)i, 1, 1.0);
       pp = sum(CLMI_PI_S, (int)i, 1, std::pow(a, 2 * al - 6));
-      saveSpectra("scalar_" + std::to_string(i), t, k, {pf, pp}, {"PS(phi)", "PS(pi)"});
+      saveSpectra("scalar_" + std::to_string(i), t, k, kavg, mult, {pf, pp}, {"PS(phi)", "PS(pi)"});
     }
     for (size_t i = 0; i < Model::NCs; ++i) {
       pf = sum(CLMI_FLD_CS, (int)i, 2, 1.0);
       pp = sum(CLMI_PI_CS, (int)i, 2, std::pow(a, 2 * al - 6));
-      saveSpectra("norm_cmplx_scalar_" + std::to_string(i), t, k, {pf, pp}, {"PS(|phi|)", "PS(|pi|)"});
+      saveSpectra("norm_cmplx_scalar_" + std::to_string(i), t, k, kavg, mult, {pf, pp}, {"PS(|phi|)", "PS(|pi|)"});
     }
     for (size_t i = 0; i < Model::NU1; ++i) {
       const auto pe = sum(CLMI_PI_U1, (int)i, 3, std::pow(a, 2 * al - 2));
       const auto pb = sum(CLMI_MAG_U1, (int)i, 3, 1.0);
-      saveSpectra("norm_U1_" + std::to_string(i), t, k, {pe, pb}, {"PS(E)", "PS(B)"});
+      saveSpectra("norm_U1_" + std::to_string(i), t, k, kavg, mult, {pe, pb}, {"PS(E)", "PS(B)"});
     }
   }
 
   bool expansion, printHeaders;
   double dk;
   int psType, psVersion;
+  int verbosity;  // spectraVerbosity: which bin columns the spectra files carry
   std::string dir;
   AverageFile energyCons, scaleFactor, energies;
   std::vector<AverageFile> means, csMeans, u1Means, gauss;

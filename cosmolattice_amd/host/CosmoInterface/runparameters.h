@@ -69,6 +69,7 @@ public:
         printHeaders(par.get<bool>("print_headers", false)),
         powerSpectrumType(par.get<int>("PS_type", 1)),
         powerSpectrumVersion(par.get<int>("PS_version", 1)),
+        spectraVerbosity(par.get<int>("spectraVerbosity", 0)),
         withGWs(par.get<bool>("withGWs", false)),
         precision(precisionFrom(par.get<std::string>("clmi_precision", "f64"))),
         device(par.get<int>("clmi_device", 0))
@@ -97,6 +98,9 @@ public:
     if (powerSpectrumVersion < 1 || powerSpectrumVersion > 3)
       throw RunParametersInconsistent("powerSpectrumVersion " + std::to_string(powerSpectrumVersion) +
                                       " is not a valid powerSpectrumVersion.");
+    if (spectraVerbosity < 0 || spectraVerbosity > 2)
+      throw RunParametersInconsistent("spectraVerbosity " + std::to_string(spectraVerbosity) +
+                                      " is not a valid spectraVerbosity.");
   }
 
   LatticeParameters<T> getLatParams() const { return LatticeParameters<T>(dx, lSide, kIR, N); }
@@ -114,6 +118,7 @@ public:
   EvolverType eType;
   bool printHeaders;
   int powerSpectrumType, powerSpectrumVersion;
+  int spectraVerbosity;  // columns of the spectra files (runparameters.h:96,233-235): 0, 1 or 2
   bool withGWs;
   int precision;  // CLMI_F64 or CLMI_F32_BULK (key clmi_precision = f64 | f32_bulk)
   int device;     // HIP device ordinal (key clmi_device)

@@ -382,3 +382,49 @@ def test_host_backup_and_restart(gpu, tmp_path):
         assert np.max(np.abs(a[k] - b[k])) <= 1e-11 * np.max(np.abs(a[k])), k
     assert bga["aI"] == pytest.approx(bgb["aI"], rel=1e-12)
     assert json.loads((rest / "end" / "state.json").read_text())["tSave"] == pytest.approx(60 * dt)
+
+
+def test_host_spectra_verbosity_columns(gpu, tmp_path):
+    """spectraVerbosity (runparameters.h:96,233-235; spectrumsaverstd.h:56-82): three runs of the lphi4
+    program from the same seed write the same spectra with 0 -> (k, PS...), 1 -> (mean |k| of the bin,
+    PS...), 2 -> (k, mean |k|, mode count, PS...).  The mean |k| and the counts equal a brute-force
+    enumeration of the lattice modes by |n| shell (the counts add up to N^3), the spectrum columns are the
+    same numbers in all three files, and spectraVerbosity = 3 is refused with the reference's message."""
+    say("case host spectra verbosity")
+    N, dx, dt, steps, every = 32, 0.6, 0.005, 4, 2
+    out = {}
+    for v in (0, 1, 2):
+        d = tmp_path / f"v{v}"
+        d.mkdir()
+        extra = f"PS_type = 1\nPS_version = 1\nbaseSeed = 5\nkCutOff = 2.0\nspectraVerbosity = {v}\n"
+        run_program("lphi4", d, lphi4_params(N, dx, dt, steps, every, extra=extra))
+        out[v] = _spectrum_blocks(d / "spectra_scalar_0.txt")
+        header = (d / "spectra_scalar_0.txt").read_text().splitlines()[0]
+        assert header.startswith({0: "#k ", 1: "#k_avg", 2: "#k    k_avg    multiplicity"}[v]), header
+    nb = out[0][0].shape[0]
+    kIR = 2 * math.pi / (N * dx)
+    n = np.fft.fftfreq(N, d=1.0 / N)
+    n = np.where(n == -N // 2, N // 2, n)
+    nn = np.sqrt(n[:, None, None] ** 2 + n[None, :, None] ** 2 + n[None, None, :] ** 2).ravel()
+    b = np.minimum(np.floor(nn + 0.5).astype(int), nb - 1)
+    count = np.bincount(b, minlength=nb).astype(float)
+    kavg = np.bincount(b, weights=nn, minlength=nb) / np.maximum(count, 1) * kIR
+    assert count.sum() == N ** 3
+    assert len(out[0]) == len(out[1]) == len(out[2]) >= 2
+    for b0, b1, b2 in zip(out[0], out[1], out[2]):
+        assert b0.shape == (nb, 3) and b1.shape == (nb, 3) and b2.shape == (nb, 5)
+        np.testing.assert_allclose(b0[:, 0], np.arange(nb) * kIR, rtol=1e-12)
+        np.testing.assert_array_equal(b2[:, 0], b0[:, 0])
+        np.testing.assert_allclose(b1[:, 0], kavg, rtol=1e-12)
+        np.testing.assert_array_equal(b2[:, 1], b1[:, 0])
+        np.testing.assert_array_equal(b2[:, 2], count)
+        for c in (1, 2):
+            np.testing.assert_array_equal(b1[:, c], b0[:, c])
+            np.testing.assert_array_equal(b2[:, 2 + c], b0[:, c])
+    d = tmp_path / "bad"
+    d.mkdir()
+    pf = d / "run.in"
+    pf.write_text(lphi4_params(N, dx, dt, steps, every, extra="spectraVerbosity = 3\n"))
+    r = subprocess.run([str(BIN / "lphi4"), f"input={pf}", f"outputfile={d}/"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "spectraVerbosity 3 is not a valid spectraVerbosity." in (r.stderr + r.stdout)
