@@ -354,29 +354,51 @@ def measured_traffic(kernel: str, f32: bool):
     return None
 
 
-def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000):
+def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000, every=250):
     """Charged scalar + U(1) (lphi4U1, e^2/lambda = 10, program units lambda = 1) on 256^3 with the
-    Gauss-consistent vacuum initial conditions: site-updates/s and HBM fraction of the fused pass in
-    both precision modes, and the Gauss-constraint violation at the end."""
+    Gauss-consistent vacuum initial conditions and self-consistent expansion, `steps` leapfrog steps in
+    both precision modes from the same initial state: site-updates/s (the evolve calls only; the checks
+    every `every` steps are outside the timed region) and HBM fraction of the fused pass, and at every
+    check the Friedmann (energy-constraint) violation (hubbleconstraint.h:27-36) and the Gauss violation
+    (gausslaws.h:31-53), with FP32-bulk's differences from FP64."""
     from cosmolattice_amd.clmi import lphi4U1
 
     spec = lphi4U1(10.0, phi0_gev=5.6964e18, lam=1.0, pi0_gev2=0.0)
     dx = 2 * math.pi / kIR / N
-    out = {"config": f"lphi4U1 {N}^3 leapfrog, e^2/lambda = 10, kIR = {kIR}, dt = {dt}, no expansion",
-           "steps": steps}
+    out = {"config": f"lphi4U1 {N}^3 leapfrog, e^2/lambda = 10, kIR = {kIR}, dt = {dt}, self-consistent expansion",
+           "steps": steps, "check_every": every}
+    recs = {}
     for name, prec in (("f64", clmi.CLMI_F64), ("f32_bulk", clmi.CLMI_F32_BULK)):
-        lat = clmi.Lattice(spec, N, dx, dt, prec, expansion=False)
+        lat = clmi.Lattice(spec, N, dx, dt, prec, expansion=True)
+        lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=0.0)
         lat.grf_u1(seed, 1, spec.cs_mass2(spec.fld0), 0.0, kIR, N * kIR / 4, spec.fld0, spec.pi0)
-        lat.set_all_averages()
+        _, _, rhs = lat.hubble_constraint()
+        lat.set_background(aDotI=math.sqrt(rhs))
         lat.lf_init()
-        sec, kms, launches, bpl = timed_run(lat, 10, steps)
+        sec, kms, launches = 0.0, 0.0, 0
+        fried, gauss = [], []
+        done = 0
+        while done < steps:
+            n = min(every, steps - done)
+            s_, k_, l_, bpl = timed_run(lat, 0, n)
+            sec, kms, launches, done = sec + s_, kms + k_, launches + l_, done + n
+            lat.lf_sync()
+            fried.append(lat.hubble_constraint()[0])
+            gauss.append(lat.gauss_u1()[0])
         avg_s = kms / 1e3 / max(launches, 1)
-        g = lat.gauss_u1()
+        recs[name] = (np.array(fried), np.array(gauss), lat.background()["aI"])
         out[name] = {"site_updates_per_s": N ** 3 * steps / sec, "ms_per_step": 1e3 * sec / steps,
                      "avg_launch_us": avg_s * 1e6, "bytes_per_launch": bpl,
-                     "hbm_frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS, "gauss_violation": g[0]}
-        log(f"[bench] u1 {name}: {out[name]['ms_per_step']:.4f} ms/step, frac {out[name]['hbm_frac']:.3f}")
+                     "hbm_frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS,
+                     "max_abs_friedmann": float(np.max(np.abs(fried))), "end_friedmann": float(fried[-1]),
+                     "max_gauss_violation": float(np.max(gauss)), "end_gauss_violation": float(gauss[-1]),
+                     "aI_end": recs[name][2]}
+        log(f"[bench] u1 {name}: {out[name]['ms_per_step']:.4f} ms/step, frac {out[name]['hbm_frac']:.3f}, "
+            f"max |Friedmann| {out[name]['max_abs_friedmann']:.3e}, max Gauss {out[name]['max_gauss_violation']:.3e}")
         lat.close()
+    (f64, g64, a64), (f32, g32, a32) = recs["f64"], recs["f32_bulk"]
+    out["f32_bulk_vs_f64"] = {"max_abs_friedmann_diff": float(np.max(np.abs(f32 - f64))),
+                              "aI_rel_diff": abs(a32 - a64) / a64}
     return out
 
 
@@ -479,7 +501,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--u1-steps", type=int, default=500)
+    ap.add_argument("--u1-steps", type=int, default=2000)
     ap.add_argument("--no-u1", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 VV2 GPU-vs-oracle run")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
