@@ -13,6 +13,7 @@
 // Global sums that feed the background adjoint are written as per-block partial rows of kAdjRed
 // doubles and combined in a fixed order (k_adj_finish), so results are reproducible.
 #include "clmi_internal.hpp"
+#include "scalar_poly.hpp"
 
 namespace clmi {
 namespace {
@@ -45,41 +46,54 @@ __device__ __forceinline__ Nbr neighbours(uint32_t s, uint32_t N)
   return n;
 }
 
+// Potential monomials m_t = prod_f phi_f^e_tf and their derivatives.  The field count NF is a template
+// parameter so that per-site arrays stay in registers; with PK > 0 the exponent table is
+// scalar_poly.hpp's compile-time PolyTable and every loop over terms and powers unrolls.
+template <int NF, int PK> __device__ __forceinline__ int n_terms(const ScalarPoly &P)
+{
+  if constexpr (PK == 0) return P.nterms;
+  else return PolyTable<NF, PK>::nt;
+}
+// Static bound of the term loops: unrolled with a guard t < n_terms, so that the per-term sums index
+// their accumulators with constants and stay in registers.
+template <int NF, int PK> constexpr int max_terms()
+{
+  if constexpr (PK == 0) return CLMI_MAX_TERMS;
+  else return PolyTable<NF, PK>::nt;
+}
+template <int NF, int PK> __device__ __forceinline__ int expo(const ScalarPoly &P, int t, int f)
+{
+  if constexpr (PK == 0) return P.e[t][f];
+  else return PolyTable<NF, PK>::e[t][f];
+}
 __device__ __forceinline__ double ipow(double v, int e)
 {
   double r = 1.0;
   for (int i = 0; i < e; ++i) r *= v;
   return r;
 }
-// monomial t and its first and second derivatives
-__device__ __forceinline__ double mono(const ScalarPoly &P, int t, const double *phi, int nf)
+// prod_f phi_f^(e_tf - [f == n] - [f == m]) (n, m = -1: no derivative)
+template <int NF, int PK> __device__ __forceinline__ double term(const ScalarPoly &P, int t, const double *phi, int n, int m)
 {
   double p = 1.0;
-  for (int f = 0; f < nf; ++f) p *= ipow(phi[f], P.e[t][f]);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) p *= ipow(phi[f], expo<NF, PK>(P, t, f) - (f == n) - (f == m));
   return p;
 }
-__device__ __forceinline__ double dmono(const ScalarPoly &P, int t, const double *phi, int nf, int n)
+template <int NF, int PK> __device__ __forceinline__ double mono(const ScalarPoly &P, int t, const double *phi)
 {
-  const int e = P.e[t][n];
-  if (e == 0) return 0.0;
-  double p = (double)e;
-  for (int f = 0; f < nf; ++f) p *= ipow(phi[f], f == n ? e - 1 : P.e[t][f]);
-  return p;
+  return term<NF, PK>(P, t, phi, -1, -1);
 }
-__device__ __forceinline__ double d2mono(const ScalarPoly &P, int t, const double *phi, int nf, int n, int m)
+template <int NF, int PK> __device__ __forceinline__ double dmono(const ScalarPoly &P, int t, const double *phi, int n)
 {
-  if (n == m) {
-    const int e = P.e[t][n];
-    if (e < 2) return 0.0;
-    double p = (double)(e * (e - 1));
-    for (int f = 0; f < nf; ++f) p *= ipow(phi[f], f == n ? e - 2 : P.e[t][f]);
-    return p;
-  }
-  const int en = P.e[t][n], em = P.e[t][m];
-  if (en == 0 || em == 0) return 0.0;
-  double p = (double)(en * em);
-  for (int f = 0; f < nf; ++f) p *= ipow(phi[f], f == n ? en - 1 : (f == m ? em - 1 : P.e[t][f]));
-  return p;
+  const int e = expo<NF, PK>(P, t, n);
+  return e == 0 ? 0.0 : (double)e * term<NF, PK>(P, t, phi, n, -1);
+}
+template <int NF, int PK>
+__device__ __forceinline__ double d2mono(const ScalarPoly &P, int t, const double *phi, int n, int m)
+{
+  const int en = expo<NF, PK>(P, t, n), em = expo<NF, PK>(P, t, m) - (n == m);
+  return (en <= 0 || em <= 0) ? 0.0 : (double)(en * em) * term<NF, PK>(P, t, phi, n, m);
 }
 
 // (A) fields adjoint of the storeFieldsAverages reductions, optionally with the momentum
@@ -87,27 +101,35 @@ __device__ __forceinline__ double d2mono(const ScalarPoly &P, int t, const doubl
 //   lam_phi_f += lamG (-2/N^3) L(phi_f) + lamU V'_f(phi) / N^3
 //   lam_pi_f  += lamP (2/N^3) pi_f                        (when lamP != 0 or with_pi)
 //   red[0] = sum lam_phi . pi,  red[1 + t] = sum m_t(phi)
-template <bool F32>
+template <int NF, bool F32, int PK>
 __global__ __launch_bounds__(kBlock) void k_adj_fields(const AdjArgs a, double lamG, double lamU, double lamP,
                                                        int with_pi)
 {
-  const int nf = a.nf;
+  if (a.sc) {
+    lamG = a.sc->lam_G;
+    lamU = a.sc->lam_U;
+  }
+  const int nt = n_terms<NF, PK>(a.poly);
   double acc[kAdjRed];
 #pragma unroll
   for (int k = 0; k < kAdjRed; ++k) acc[k] = 0.0;
   const double invV = 1.0 / (double)a.V;
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
     const Nbr nb = neighbours(s, a.N);
-    double phi[CLMI_MAX_SCALARS];
-    for (int f = 0; f < nf; ++f) phi[f] = ld_phi<F32>(a.fwd, f, s);
-    for (int f = 0; f < nf; ++f) {
+    double phi[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) phi[f] = ld_phi<F32>(a.fwd, f, s);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
       const double c = phi[f];
       const double lap = (ld_phi<F32>(a.fwd, f, nb.xp) + ld_phi<F32>(a.fwd, f, nb.xm) - 2 * c +
                           ld_phi<F32>(a.fwd, f, nb.yp) + ld_phi<F32>(a.fwd, f, nb.ym) - 2 * c +
                           ld_phi<F32>(a.fwd, f, nb.zp) + ld_phi<F32>(a.fwd, f, nb.zm) - 2 * c) *
                          a.inv_dx2;
       double dv = 0.0;
-      for (int t = 0; t < a.poly.nterms; ++t) dv += a.poly.c[t] * dmono(a.poly, t, phi, nf, f);
+#pragma unroll
+      for (int t = 0; t < max_terms<NF, PK>(); ++t)
+        if (t < nt) dv += a.poly.c[t] * dmono<NF, PK>(a.poly, t, phi, f);
       double l = a.lam_phi[f][s];
       l += lamG * (-2.0 * invV) * lap + lamU * dv * invV;
       a.lam_phi[f][s] = l;
@@ -115,63 +137,99 @@ __global__ __launch_bounds__(kBlock) void k_adj_fields(const AdjArgs a, double l
       acc[0] += l * p;
       if (with_pi) a.lam_pi[f][s] += lamP * 2.0 * invV * p;
     }
-    for (int t = 0; t < a.poly.nterms && t + 1 < kAdjRed; ++t) acc[1 + t] += mono(a.poly, t, phi, nf);
+#pragma unroll
+    for (int t = 0; t < max_terms<NF, PK>(); ++t)
+      if (t < nt && t + 1 < kAdjRed) acc[1 + t] += mono<NF, PK>(a.poly, t, phi);
   }
   block_reduce_store<kAdjRed>(acc, a.partials + (size_t)blockIdx.x * kAdjRed);
 }
 
 // (B) drift and <pi^2> transposes (pointwise): lam_pi_f += cF lam_phi_f + lamP (2/N^3) pi_f
-template <bool F32> __global__ __launch_bounds__(kBlock) void k_adj_pi(const AdjArgs a, double cF, double lamP)
+template <int NF, bool F32> __global__ __launch_bounds__(kBlock) void k_adj_pi(const AdjArgs a, double cF, double lamP)
 {
+  if (a.sc) {
+    cF = a.sc->cF;
+    lamP = a.sc->lamP;
+  }
   const double c2 = lamP * 2.0 / (double)a.V;
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock)
-    for (int f = 0; f < a.nf; ++f) a.lam_pi[f][s] += cF * a.lam_phi[f][s] + c2 * ld_pi<F32>(a.fwd, f, s);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) a.lam_pi[f][s] += cF * a.lam_phi[f][s] + c2 * ld_pi<F32>(a.fwd, f, s);
 }
 
 // (C) kick transpose on the fields of step n:
 //   lam_phi_f += c [aK L(lam_pi_f) - aV sum_g H_fg(phi) lam_pi_g]
 //   red[0] = sum lam_pi . L(phi),  red[1] = sum lam_pi . V'(phi),
 //   red[2 + t] = sum_f lam_pi_f d m_t / d phi_f
-template <bool F32>
+// FIELDS: then the fields transpose (A) of the same state, with the Laplacian and V' the kick already
+// evaluated, in the order (A) applies them:
+//   lam_phi_f += lamG (-2/N^3) L(phi_f) + lamU V'_f(phi) / N^3
+//   red[F] = sum lam_phi . pi,  red[F + 1 + t] = sum m_t(phi),  F = kAdjFieldSlots
+template <int NF, bool F32, bool FIELDS, int PK>
 __global__ __launch_bounds__(kBlock) void k_adj_kick(const AdjArgs a, double c, double aK, double aV)
 {
-  const int nf = a.nf;
+  double lamG = 0.0, lamU = 0.0;
+  if (a.sc) {
+    c = a.sc->ck;
+    aK = a.sc->aK;
+    aV = a.sc->aV;
+    lamG = a.sc->fG;
+    lamU = a.sc->fU;
+  }
+  const int nt = n_terms<NF, PK>(a.poly);
+  const double invV = 1.0 / (double)a.V;
   double acc[kAdjRed];
 #pragma unroll
   for (int k = 0; k < kAdjRed; ++k) acc[k] = 0.0;
   for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < a.V; s += gridDim.x * kBlock) {
     const Nbr nb = neighbours(s, a.N);
-    double phi[CLMI_MAX_SCALARS], lp[CLMI_MAX_SCALARS];
-    for (int f = 0; f < nf; ++f) {
+    double phi[NF], lp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
       phi[f] = ld_phi<F32>(a.fwd, f, s);
       lp[f] = a.lam_pi[f][s];
     }
-    for (int f = 0; f < nf; ++f) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
       const double *L = a.lam_pi[f];
       const double lapl = (L[nb.xp] + L[nb.xm] - 2 * lp[f] + L[nb.yp] + L[nb.ym] - 2 * lp[f] + L[nb.zp] + L[nb.zm] -
                            2 * lp[f]) *
                           a.inv_dx2;
       double hl = 0.0;
-      for (int g = 0; g < nf; ++g) {
+#pragma unroll
+      for (int g = 0; g < NF; ++g) {
         double h = 0.0;
-        for (int t = 0; t < a.poly.nterms; ++t) h += a.poly.c[t] * d2mono(a.poly, t, phi, nf, f, g);
+#pragma unroll
+        for (int t = 0; t < max_terms<NF, PK>(); ++t)
+          if (t < nt) h += a.poly.c[t] * d2mono<NF, PK>(a.poly, t, phi, f, g);
         hl += h * lp[g];
       }
-      a.lam_phi[f][s] += c * (aK * lapl - aV * hl);
+      double l = a.lam_phi[f][s] + c * (aK * lapl - aV * hl);
       const double c0 = phi[f];
       const double lapf = (ld_phi<F32>(a.fwd, f, nb.xp) + ld_phi<F32>(a.fwd, f, nb.xm) - 2 * c0 +
                            ld_phi<F32>(a.fwd, f, nb.yp) + ld_phi<F32>(a.fwd, f, nb.ym) - 2 * c0 +
                            ld_phi<F32>(a.fwd, f, nb.zp) + ld_phi<F32>(a.fwd, f, nb.zm) - 2 * c0) *
                           a.inv_dx2;
       double dv = 0.0;
-      for (int t = 0; t < a.poly.nterms; ++t) {
-        const double dm = dmono(a.poly, t, phi, nf, f);
+#pragma unroll
+      for (int t = 0; t < max_terms<NF, PK>(); ++t) {
+        if (t >= nt) break;
+        const double dm = dmono<NF, PK>(a.poly, t, phi, f);
         dv += a.poly.c[t] * dm;
-        if (t + 2 < kAdjRed) acc[2 + t] += lp[f] * dm;
+        if (t + 2 < kAdjFieldSlots) acc[2 + t] += lp[f] * dm;
       }
       acc[0] += lp[f] * lapf;
       acc[1] += lp[f] * dv;
+      if (FIELDS) {
+        l += lamG * (-2.0 * invV) * lapf + lamU * dv * invV;
+        acc[kAdjFieldSlots] += l * ld_pi<F32>(a.fwd, f, s);
+      }
+      a.lam_phi[f][s] = l;
     }
+    if (FIELDS)
+#pragma unroll
+      for (int t = 0; t < max_terms<NF, PK>(); ++t)
+        if (t < nt && kAdjFieldSlots + 1 + t < kAdjRed) acc[kAdjFieldSlots + 1 + t] += mono<NF, PK>(a.poly, t, phi);
   }
   block_reduce_store<kAdjRed>(acc, a.partials + (size_t)blockIdx.x * kAdjRed);
 }
@@ -197,7 +255,76 @@ __global__ __launch_bounds__(kBlock) void k_adj_finish(const double *partials, i
   }
 }
 
+// The host arithmetic of the reverse sweep's background chain, moved to the device so that the sweep
+// needs no host round trip (see launch_adj_chain_fields / launch_adj_chain_kick).
+__global__ void k_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
+                                   double w, double V, const ModelConsts mc, int nterms, int ex)
+{
+  if (threadIdx.x != 0) return;
+  const double dt = mc.dt, al = mc.alpha;
+  const clmi_background &bn = tape[n], &bn1 = tape[n + 1];
+  for (int t = 0; t < nterms; ++t) S->gc[t] += S->lam_U * red[fo + 1 + t] / V;
+  const double F = pow(bn1.aSI, al - 3);
+  double lamP = 0.0;
+  S->fG = S->lam_G;
+  S->fU = S->lam_U;
+  if (ex) {
+    // drift: aSI' = (aI + aI')/2 enters F; aI' = aI + dt aDotSI'; aDotSI' = aDotSI + dt K
+    const double lam_aSI1 = dt * red[fo] * (al - 3) * pow(bn1.aSI, al - 4);
+    const double lam_aI1 = S->lam_aI + 0.5 * lam_aSI1;
+    double n_aI = 0.5 * lam_aSI1 + lam_aI1;
+    const double lam_aDotSI1 = S->lam_aDotSI + dt * lam_aI1;
+    const double lamK = dt * lam_aDotSI1;
+    const SFd d = sf_partials(mc, bn.aI, bn1.pi2AvI, bn.grad2AvI, bn.potAvI);
+    const double lam_pi2I = lamK * d.dpi2;
+    n_aI += lamK * d.da;
+    S->n_aI = n_aI;
+    S->n_aDotSI = lam_aDotSI1;
+    S->n_G = lamK * d.dG;
+    S->n_U = lamK * d.dU;
+    S->fG = S->n_G;
+    S->fU = S->n_U;
+    // pi2AvI = (pi2AvSI_old + P)/2, pi2AvSI' = P
+    S->n_pi2SI = 0.5 * lam_pi2I;
+    lamP = 0.5 * lam_pi2I + S->lam_pi2SI;
+  }
+  S->cF = dt * F;
+  S->lamP = lamP;
+  S->ck = w * dt;
+  S->aK = pow(bn.aI, 1 + al);
+  S->aV = pow(bn.aI, 3 + al);
+}
+
+__global__ void k_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
+                                 const ModelConsts mc, int nterms, int ex)
+{
+  if (threadIdx.x != 0) return;
+  const double al = mc.alpha, aI = tape[n].aI, ck = S->ck, aV = S->aV;
+  for (int t = 0; t < nterms; ++t) S->gc[t] += -ck * aV * red[2 + t];
+  if (ex) {
+    S->lam_aI = S->n_aI + ck * (red[0] * (1 + al) * pow(aI, al) - red[1] * (3 + al) * pow(aI, 2 + al));
+    S->lam_aDotSI = S->n_aDotSI;
+    S->lam_pi2SI = S->n_pi2SI;
+    S->lam_G = S->n_G;
+    S->lam_U = S->n_U;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
+                                   double w, double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adj_chain_fields, dim3(1), dim3(64), 0, s, S, red, fo, tape, n, w, V, mc, nterms, (int)ex);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
+                                 const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adj_chain_kick, dim3(1), dim3(64), 0, s, S, red, tape, n, mc, nterms, (int)ex);
+  return hipGetLastError();
+}
 
 int adj_blocks(uint32_t V)
 {
@@ -205,41 +332,72 @@ int adj_blocks(uint32_t V)
   return (int)(b < 1024 ? b : 1024);
 }
 
+namespace {
+
+// Dispatch over the field count (1..4), the storage precision and the potential form (compile-time
+// lphi4 table for 1 or 2 fields when the run's table is lphi4's).
+template <template <int, bool, int> class K, class... Args>
+hipError_t adj_dispatch(const AdjArgs &a, bool f32, hipStream_t s, Args... args)
+{
+  const int blocks = adj_blocks(a.V);
+  const int pk = (a.nf <= 2) ? poly_kind(a.poly, a.nf) : 0;
+#define CLMI_ADJ(NF, F, PK)                                                                                     \
+  if (a.nf == NF && f32 == F && pk == PK) {                                                                     \
+    hipLaunchKernelGGL((K<NF, F, PK>::kernel), dim3(blocks), dim3(kBlock), 0, s, a, args...);                   \
+    return hipGetLastError();                                                                                   \
+  }
+  CLMI_ADJ(1, false, 0) CLMI_ADJ(1, true, 0) CLMI_ADJ(1, false, 1) CLMI_ADJ(1, true, 1)
+  CLMI_ADJ(2, false, 0) CLMI_ADJ(2, true, 0) CLMI_ADJ(2, false, 1) CLMI_ADJ(2, true, 1)
+  CLMI_ADJ(3, false, 0) CLMI_ADJ(3, true, 0) CLMI_ADJ(4, false, 0) CLMI_ADJ(4, true, 0)
+#undef CLMI_ADJ
+  return hipErrorInvalidValue;
+}
+
+template <int NF, bool F32, int PK> struct FieldsK {
+  static constexpr auto kernel = k_adj_fields<NF, F32, PK>;
+};
+template <int NF, bool F32, int PK> struct PiK {
+  static constexpr auto kernel = k_adj_pi<NF, F32>;
+};
+template <int NF, bool F32, int PK> struct KickK {
+  static constexpr auto kernel = k_adj_kick<NF, F32, false, PK>;
+};
+template <int NF, bool F32, int PK> struct KickFieldsK {
+  static constexpr auto kernel = k_adj_kick<NF, F32, true, PK>;
+};
+
+hipError_t adj_finish(const AdjArgs &a, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, adj_blocks(a.V), a.red_out);
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lamU, double lamP, bool with_pi,
                              hipStream_t s)
 {
-  const int blocks = adj_blocks(a.V);
-  if (f32)
-    hipLaunchKernelGGL(k_adj_fields<true>, dim3(blocks), dim3(kBlock), 0, s, a, lamG, lamU, lamP, (int)with_pi);
-  else
-    hipLaunchKernelGGL(k_adj_fields<false>, dim3(blocks), dim3(kBlock), 0, s, a, lamG, lamU, lamP, (int)with_pi);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, blocks, a.red_out);
-  return hipGetLastError();
+  hipError_t e = adj_dispatch<FieldsK>(a, f32, s, lamG, lamU, lamP, (int)with_pi);
+  return e == hipSuccess ? adj_finish(a, s) : e;
 }
 
 hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s)
 {
-  const int blocks = adj_blocks(a.V);
-  if (f32)
-    hipLaunchKernelGGL(k_adj_pi<true>, dim3(blocks), dim3(kBlock), 0, s, a, cF, lamP);
-  else
-    hipLaunchKernelGGL(k_adj_pi<false>, dim3(blocks), dim3(kBlock), 0, s, a, cF, lamP);
-  return hipGetLastError();
+  return adj_dispatch<PiK>(a, f32, s, cF, lamP);
 }
 
 hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s)
 {
-  const int blocks = adj_blocks(a.V);
-  if (f32)
-    hipLaunchKernelGGL(k_adj_kick<true>, dim3(blocks), dim3(kBlock), 0, s, a, c, aK, aV);
-  else
-    hipLaunchKernelGGL(k_adj_kick<false>, dim3(blocks), dim3(kBlock), 0, s, a, c, aK, aV);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, blocks, a.red_out);
-  return hipGetLastError();
+  hipError_t e = adj_dispatch<KickK>(a, f32, s, c, aK, aV);
+  return e == hipSuccess ? adj_finish(a, s) : e;
+}
+
+hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, hipStream_t s)
+{
+  if (!a.sc) return hipErrorInvalidValue;
+  hipError_t e = with_fields ? adj_dispatch<KickFieldsK>(a, f32, s, 0.0, 0.0, 0.0)
+                             : adj_dispatch<KickK>(a, f32, s, 0.0, 0.0, 0.0);
+  return e == hipSuccess ? adj_finish(a, s) : e;
 }
 
 }  // namespace clmi

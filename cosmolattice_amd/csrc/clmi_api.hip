@@ -1236,25 +1236,6 @@ AdjFields adj_view(const clmi_ctx *c, const AdjState &s)
   return v;
 }
 
-// ScaleFactorKernels::get for scalar singlets and its partial derivatives.
-struct SFd {
-  double K, dpi2, dG, dU, da;
-};
-SFd sf_partials(const ModelConsts &mc, double a, double pi2, double G, double U)
-{
-  const double al = mc.alpha, fm = mc.fStar / mc.MPl;
-  const double A = std::pow(a, 2 * al + 1) / 3.0 * fm * fm;
-  const double B = (al - 2) * 0.5 * pi2 * std::pow(a, -6.0) + al * 0.5 * G * std::pow(a, -2.0) + (al + 1) * U;
-  SFd d;
-  d.K = A * B;
-  d.dpi2 = A * (al - 2) * 0.5 * std::pow(a, -6.0);
-  d.dG = A * al * 0.5 * std::pow(a, -2.0);
-  d.dU = A * (al + 1);
-  d.da = (2 * al + 1) / a * A * B +
-         A * ((al - 2) * 0.5 * pi2 * (-6.0) * std::pow(a, -7.0) + al * 0.5 * G * (-2.0) * std::pow(a, -3.0));
-  return d;
-}
-
 struct AdjRun {
   clmi_ctx *c = nullptr;
   std::vector<AdjState> checkpoints, segment;
@@ -1262,6 +1243,8 @@ struct AdjRun {
   double *lam = nullptr;       // 2 * n_scalars FP64 arrays
   double *partials = nullptr;  // 1024 x kAdjRed
   double *red = nullptr;       // kAdjRed
+  AdjScalars *sc = nullptr;    // background adjoint during the reverse sweep
+  clmi_background *tape = nullptr;  // device copy of the forward background after every step
   ~AdjRun()
   {
     for (auto &s : checkpoints) adj_free(s);
@@ -1270,6 +1253,8 @@ struct AdjRun {
     if (lam) (void)hipFree(lam);
     if (partials) (void)hipFree(partials);
     if (red) (void)hipFree(red);
+    if (sc) (void)hipFree(sc);
+    if (tape) (void)hipFree(tape);
   }
 };
 
@@ -1324,6 +1309,8 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   HIPCHK(c, hipMalloc(&run.lam, sizeof(double) * 2 * nf * c->V));
   HIPCHK(c, hipMalloc(&run.partials, sizeof(double) * 1024 * kAdjRed));
   HIPCHK(c, hipMalloc(&run.red, sizeof(double) * kAdjRed));
+  HIPCHK(c, hipMalloc(&run.sc, sizeof(AdjScalars)));
+  HIPCHK(c, hipMalloc(&run.tape, sizeof(clmi_background) * (size_t)(nsteps + 1)));
 
   // ---- forward: initial background, lf_init, nsteps steps, sync; record the background tape ----
   std::vector<clmi_background> tape(nsteps + 1);
@@ -1347,6 +1334,8 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   CHK(clmi_lf_sync(c));
   CHK(clmi_get_background(c, &b_fin));
   CHK(adj_save(c, run.final_state));
+  HIPCHK(c, hipMemcpyAsync(run.tape, tape.data(), sizeof(clmi_background) * (size_t)(nsteps + 1),
+                           hipMemcpyHostToDevice, c->stream));
 
   // ---- reverse ----
   AdjArgs a{};
@@ -1392,6 +1381,24 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
     for (int t = 0; t < nt; ++t) gc[t] += -0.5 * dt * aV * red[2 + t];
   }
 
+  // The background adjoint lives on the device for the sweep: one-thread chain kernels between the
+  // lattice kernels replace the host arithmetic, so no step waits for the host.
+  {
+    AdjScalars h{};
+    h.lam_aI = lam_aI;
+    h.lam_aDotSI = lam_aDotSI;
+    h.lam_pi2SI = lam_pi2SI;
+    h.lam_G = lam_G;
+    h.lam_U = lam_U;
+    for (int t = 0; t < nt; ++t) h.gc[t] = gc[t];
+    HIPCHK(c, hipMemcpyAsync(run.sc, &h, sizeof(AdjScalars), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  // Per reverse step n: the pi transpose of step n, then one pass with the kick transpose of step n
+  // fused with the fields transpose of state n (which the original order runs first thing in step
+  // n - 1), each followed by its scalar chain.  The fields transpose of the final state runs alone.
+  AdjArgs ad = a;
+  ad.sc = run.sc;
   for (int k = ncp - 1; k >= 0; --k) {
     const int n0 = k * M, n1 = std::min(nsteps, n0 + M);
     // recompute the segment's states n0..n1 from checkpoint k
@@ -1404,50 +1411,36 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
       CHK(adj_forward_step(c));
     }
     CHK(adj_save(c, run.segment[n1 - n0]));
-    for (int n = n1 - 1; n >= n0; --n) {
-      const clmi_background &bn = tape[n], &bn1 = tape[n + 1];
-      const double w = (n == 0) ? 0.5 : 1.0;
-      // storeFieldsAverages of phi_{n+1}, then the drift's sum lam_phi . pi_{n+1/2}
-      a.fwd = adj_view(c, run.segment[n + 1 - n0]);
-      HIPCHK(c, launch_adj_fields(a, c->f32, lam_G, lam_U, 0.0, false, c->stream));
-      CHK(adj_read_red(c, run, red));
-      for (int t = 0; t < nt; ++t) gc[t] += lam_U * red[1 + t] / V;
-      const double F = std::pow(bn1.aSI, al - 3);
-      double lamP = 0.0;
-      double lam_aI_n = 0, lam_aDotSI_n = 0, lam_pi2SI_n = 0, lam_G_n = 0, lam_U_n = 0;
-      if (ex) {
-        const double lam_aSI1 = dt * red[0] * (al - 3) * std::pow(bn1.aSI, al - 4);
-        // aSI' = (aI + aI')/2; aI' = aI + dt aDotSI'; aDotSI' = aDotSI + dt K
-        double lam_aI1 = lam_aI + 0.5 * lam_aSI1;
-        lam_aI_n = 0.5 * lam_aSI1 + lam_aI1;
-        const double lam_aDotSI1 = lam_aDotSI + dt * lam_aI1;
-        lam_aDotSI_n = lam_aDotSI1;
-        const double lamK = dt * lam_aDotSI1;
-        const SFd d = sf_partials(mc, bn.aI, bn1.pi2AvI, bn.grad2AvI, bn.potAvI);
-        const double lam_pi2I = lamK * d.dpi2;
-        lam_G_n = lamK * d.dG;
-        lam_U_n = lamK * d.dU;
-        lam_aI_n += lamK * d.da;
-        // pi2AvI = (pi2AvSI_old + P)/2, pi2AvSI' = P
-        lam_pi2SI_n = 0.5 * lam_pi2I;
-        lamP = 0.5 * lam_pi2I + lam_pi2SI;
-      }
-      HIPCHK(c, launch_adj_pi(a, c->f32, dt * F, lamP, c->stream));
-      // kick of step n on phi_n
-      a.fwd = adj_view(c, run.segment[n - n0]);
-      const double aK = std::pow(bn.aI, 1 + al), aV = std::pow(bn.aI, 3 + al), ck = w * dt;
-      HIPCHK(c, launch_adj_kick(a, c->f32, ck, aK, aV, c->stream));
-      CHK(adj_read_red(c, run, red));
-      if (ex) lam_aI_n += ck * (red[0] * (1 + al) * std::pow(bn.aI, al) - red[1] * (3 + al) * std::pow(bn.aI, 2 + al));
-      for (int t = 0; t < nt; ++t) gc[t] += -ck * aV * red[2 + t];
-      if (ex) {
-        lam_aI = lam_aI_n;
-        lam_aDotSI = lam_aDotSI_n;
-        lam_pi2SI = lam_pi2SI_n;
-        lam_G = lam_G_n;
-        lam_U = lam_U_n;
-      }
+    if (k == ncp - 1) {
+      // storeFieldsAverages of the final phi, then the drift's sum lam_phi . pi
+      ad.fwd = adj_view(c, run.segment[n1 - n0]);
+      HIPCHK(c, launch_adj_fields(ad, c->f32, 0.0, 0.0, 0.0, false, c->stream));
+      HIPCHK(c, launch_adj_chain_fields(run.sc, run.red, 0, run.tape, nsteps - 1, nsteps == 1 ? 0.5 : 1.0, V, mc, nt,
+                                        ex, c->stream));
     }
+    for (int n = n1 - 1; n >= n0; --n) {
+      // drift and <pi^2> transposes of step n: pi_{n+1/2} is the momentum of state n + 1
+      ad.fwd = adj_view(c, run.segment[n + 1 - n0]);
+      HIPCHK(c, launch_adj_pi(ad, c->f32, 0.0, 0.0, c->stream));
+      // kick of step n on phi_n, with the fields transpose of phi_n for step n - 1
+      ad.fwd = adj_view(c, run.segment[n - n0]);
+      HIPCHK(c, launch_adj_kick_fields(ad, c->f32, n > 0, c->stream));
+      HIPCHK(c, launch_adj_chain_kick(run.sc, run.red, run.tape, n, mc, nt, ex, c->stream));
+      if (n > 0)
+        HIPCHK(c, launch_adj_chain_fields(run.sc, run.red, kAdjFieldSlots, run.tape, n - 1, n - 1 == 0 ? 0.5 : 1.0, V,
+                                          mc, nt, ex, c->stream));
+    }
+  }
+  {
+    AdjScalars h{};
+    HIPCHK(c, hipMemcpyAsync(&h, run.sc, sizeof(AdjScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    lam_aI = h.lam_aI;
+    lam_aDotSI = h.lam_aDotSI;
+    lam_pi2SI = h.lam_pi2SI;
+    lam_G = h.lam_G;
+    lam_U = h.lam_U;
+    for (int t = 0; t < nt; ++t) gc[t] = h.gc[t];
   }
 
   // lf_init (leapfrog.h:35): aDotSI = aDotI - dt/2 K(initial averages)

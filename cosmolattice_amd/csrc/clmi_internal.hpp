@@ -202,11 +202,24 @@ struct UpdMember {    // one lattice's background update inside a batched launch
 hipError_t launch_update_batch(const UpdMember *d_members, int n, const UpdateOps &ops, hipStream_t s);
 
 // ---- adjoint (adjoint_kernels.hip) ----
-constexpr int kAdjRed = 16;  // partial-sum slots of the adjoint reductions
+constexpr int kAdjRed = 32;  // partial-sum slots of the adjoint reductions
+constexpr int kAdjFieldSlots = 16;  // first slot of the fields-transpose sums in the fused reverse pass
 struct AdjFields {           // forward fields of one leapfrog state, in storage precision
   const void *phi[CLMI_MAX_SCALARS];
   const float *philo[CLMI_MAX_SCALARS];
   const void *pi[CLMI_MAX_SCALARS];
+};
+// Adjoint of the background chain, resident on the device during the reverse sweep of
+// clmi_adjoint_lf: the adjoints of the current background scalars, those of the step being reversed
+// (committed once its kick transpose has run), the coefficients the next lattice kernels read, and
+// the accumulated derivatives by the potential coefficients.
+struct AdjScalars {
+  double lam_aI, lam_aDotSI, lam_pi2SI, lam_G, lam_U;
+  double n_aI, n_aDotSI, n_pi2SI, n_G, n_U;
+  double cF, lamP;    // pi transpose: lam_pi += cF lam_phi + lamP (2/N^3) pi
+  double ck, aK, aV;  // kick transpose
+  double fG, fU;      // fields transpose of the kicked state (fused with the kick transpose)
+  double gc[CLMI_MAX_TERMS];
 };
 struct AdjArgs {
   AdjFields fwd;
@@ -218,11 +231,47 @@ struct AdjArgs {
   ScalarPoly poly;
   double *partials;  // >= 1024 rows of kAdjRed
   double *red_out;   // kAdjRed finished sums
+  const AdjScalars *sc;  // non-null: the kernels take their coefficients from here, not from arguments
 };
+
+// ScaleFactorKernels::get for scalar singlets (evolvers/kernels/scalefactorkernels.h:27-81) and its
+// partial derivatives by <pi^2>, <|grad phi|^2>, <V> and a.
+struct SFd {
+  double K, dpi2, dG, dU, da;
+};
+__host__ __device__ inline SFd sf_partials(const ModelConsts &mc, double a, double pi2, double G, double U)
+{
+  const double al = mc.alpha, fm = mc.fStar / mc.MPl;
+  const double A = pow(a, 2 * al + 1) / 3.0 * fm * fm;
+  const double B = (al - 2) * 0.5 * pi2 * pow(a, -6.0) + al * 0.5 * G * pow(a, -2.0) + (al + 1) * U;
+  SFd d;
+  d.K = A * B;
+  d.dpi2 = A * (al - 2) * 0.5 * pow(a, -6.0);
+  d.dG = A * al * 0.5 * pow(a, -2.0);
+  d.dU = A * (al + 1);
+  d.da = (2 * al + 1) / a * A * B + A * ((al - 2) * 0.5 * pi2 * (-6.0) * pow(a, -7.0) + al * 0.5 * G * (-2.0) * pow(a, -3.0));
+  return d;
+}
+// Scalar steps of the reverse sweep, one thread each, on the stream after the lattice kernel whose
+// sums (red) they consume.  chain_fields: after the fields transpose of state n+1 -- derivatives by
+// the potential coefficients, the background adjoint of the drift and scale-factor update of step n,
+// and the coefficients of the pi and kick transposes of step n (w: the kick weight, V: N^3; the
+// fields sums sit at red[fo], red[fo + 1 + t]).  chain_kick:
+// after the kick transpose of step n -- its a_I and coefficient terms, then the step's background
+// adjoints become the current ones.
+hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
+                                   double w, double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
+hipError_t launch_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
+                                 const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
 hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lamU, double lamP, bool with_pi,
                              hipStream_t s);
 hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s);
 hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s);
+// Kick transpose of step n fused with the fields transpose of the same state (coefficients from
+// a.sc): the sums of the kick at slots 0 .. 1 + n_terms, those of the fields transpose at
+// kAdjFieldSlots (lam_phi . pi) and kAdjFieldSlots + 1 + t (monomials).  with_fields = false: the
+// kick transpose alone.
+hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, hipStream_t s);
 
 // Launchers (defined in the .hip files).
 struct TiledPlan {

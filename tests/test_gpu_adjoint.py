@@ -25,9 +25,9 @@ BAND = (0.8, 2.4)
 
 def initial_fields(spec, rng):
     phi = np.stack([spec.fld0[f] + smooth_random(N, rng, amp=0.05) + 0.01 * rng.normal(size=(N,) * 3)
-                    for f in range(2)])
+                    for f in range(spec.n_scalars)])
     pi = np.stack([spec.pi0[f] + smooth_random(N, rng, amp=0.05) + 0.01 * rng.normal(size=(N,) * 3)
-                   for f in range(2)])
+                   for f in range(spec.n_scalars)])
     return phi, pi
 
 
@@ -176,20 +176,25 @@ def test_adjoint_f32_bulk_tracks_fp64(gpu):
         assert np.max(np.abs(a - b)) < 1e-4 * np.max(np.abs(b))
 
 
-@pytest.mark.parametrize("expansion", [True, False])
-def test_adjoint_matches_fp64_restatement(gpu, oracle_mod, expansion):
+@pytest.mark.parametrize("expansion,model", [(True, "lphi4"), (False, "lphi4"), (True, "runtime"), (True, "lphi4_1f")])
+def test_adjoint_matches_fp64_restatement(gpu, oracle_mod, expansion, model):
     """The HIP adjoint (checkpointed) equals the numpy FP64 reverse-mode restatement on the same
-    seeds: adjoint fields and coefficient gradients to 1e-9 relative."""
-    say(f"case adjoint vs restatement expansion={expansion}")
-    spec = bench_spec()
+    seeds: adjoint fields and coefficient gradients to 1e-9 relative.  The models cover every form of
+    the adjoint kernels: two-field and one-field lphi4 (compile-time exponent table) and a two-field
+    potential that is not lphi4's (runtime table, four terms)."""
+    from test_gpu_parity import runtime_poly_spec
+
+    say(f"case adjoint vs restatement expansion={expansion} model={model}")
+    spec = {"lphi4": bench_spec, "runtime": runtime_poly_spec, "lphi4_1f": lambda: bench_spec(0.0, n_fields=1)}[model]()
+    nf = spec.n_scalars
     rng = np.random.default_rng(13)
     phi0, pi0 = initial_fields(spec, rng)
     nsteps = 30
-    sf = [rng.normal(size=(N,) * 3) for _ in range(2)]
-    sp = [rng.normal(size=(N,) * 3) for _ in range(2)]
+    sf = [rng.normal(size=(N,) * 3) for _ in range(nf)]
+    sp = [rng.normal(size=(N,) * 3) for _ in range(nf)]
     sb = (0.3, -0.2) if expansion else (0.0, 0.0)
     lat = Lattice(spec, N, DX, DT, CLMI_F64, expansion=expansion)
-    for f in range(2):
+    for f in range(nf):
         lat.upload(FLD_S, f, 0, phi0[f])
         lat.upload(PI_S, f, 0, pi0[f])
     gf, gp, gc, _ = lat.adjoint_lf(nsteps, sf, sp, sb, checkpoint_every=7)
