@@ -216,6 +216,19 @@ __global__ void k_join(const float *hi, const float *lo, double *dst, uint32_t V
     dst[i] = (double)hi[i] + (lo ? (double)lo[i] : 0.0);
 }
 
+// Several arrays copied by one launch: each workgroup strides over every array in turn.
+__global__ __launch_bounds__(256) void k_copy_set(const CopySet cs)
+{
+  const size_t stride = (size_t)gridDim.x * 256;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  for (int a = 0; a < cs.count; ++a) {
+    const v4 *src = (const v4 *)cs.src[a];
+    v4 *dst = (v4 *)cs.dst[a];
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < cs.n16[a]; i += stride)
+      __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+  }
+}
+
 // Plain streaming copy (16-byte vectors), used only to calibrate hardware byte counters.
 __global__ __launch_bounds__(256) void k_copy(const float4 *src, float4 *dst, size_t n)
 {
@@ -292,6 +305,14 @@ hipError_t launch_means(const MeansArgs &a, int blocks, hipStream_t s)
 hipError_t launch_sum_rows(const double *partials, int rows, double *out, hipStream_t s)
 {
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kUpdThreads), 0, s, partials, rows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_set(const CopySet &cs, hipStream_t s)
+{
+  if (cs.count < 0 || cs.count > kCopySetMax) return hipErrorInvalidValue;
+  if (cs.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_set, dim3(2048), dim3(256), 0, s, cs);
   return hipGetLastError();
 }
 

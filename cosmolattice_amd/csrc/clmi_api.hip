@@ -13,6 +13,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -1201,30 +1202,35 @@ void adj_free(AdjState &s)
     free_arr(s.p[f]);
   }
 }
-int adj_copy(clmi_ctx *c, const DevArr &src, DevArr &dst, bool field)
+// One copy launch for a whole state (every array of every scalar singlet).
+void copy_add(CopySet &cs, const void *src, void *dst, size_t bytes)
 {
-  const size_t b = bytes_field(c->f32) * c->V;
-  HIPCHK(c, hipMemcpyAsync(dst.p, src.p, b, hipMemcpyDeviceToDevice, c->stream));
-  if (field && c->f32)
-    HIPCHK(c, hipMemcpyAsync(dst.lo, src.lo, sizeof(float) * c->V, hipMemcpyDeviceToDevice, c->stream));
-  return CLMI_OK;
+  cs.src[cs.count] = src;
+  cs.dst[cs.count] = dst;
+  cs.n16[cs.count] = bytes / 16;
+  cs.count++;
 }
-int adj_save(clmi_ctx *c, AdjState &s)
+int adj_copy_state(clmi_ctx *c, bool save, AdjState &s)
 {
+  CopySet cs{};
+  const size_t bf = bytes_field(c->f32) * c->V;
   for (int f = 0; f < c->cfg.n_scalars; ++f) {
-    CHK(adj_copy(c, c->fS[f][c->cur_fS], s.f[f], true));
-    CHK(adj_copy(c, c->pS[f][c->cur_pS], s.p[f], false));
+    DevArr &cf = c->fS[f][c->cur_fS], &cp = c->pS[f][c->cur_pS];
+    if (save) {
+      copy_add(cs, cf.p, s.f[f].p, bf);
+      if (c->f32) copy_add(cs, cf.lo, s.f[f].lo, sizeof(float) * c->V);
+      copy_add(cs, cp.p, s.p[f].p, bf);
+    } else {
+      copy_add(cs, s.f[f].p, cf.p, bf);
+      if (c->f32) copy_add(cs, s.f[f].lo, cf.lo, sizeof(float) * c->V);
+      copy_add(cs, s.p[f].p, cp.p, bf);
+    }
   }
+  HIPCHK(c, launch_copy_set(cs, c->stream));
   return CLMI_OK;
 }
-int adj_restore(clmi_ctx *c, const AdjState &s)
-{
-  for (int f = 0; f < c->cfg.n_scalars; ++f) {
-    CHK(adj_copy(c, s.f[f], c->fS[f][c->cur_fS], true));
-    CHK(adj_copy(c, s.p[f], c->pS[f][c->cur_pS], false));
-  }
-  return CLMI_OK;
-}
+int adj_save(clmi_ctx *c, AdjState &s) { return adj_copy_state(c, true, s); }
+int adj_restore(clmi_ctx *c, const AdjState &s) { return adj_copy_state(c, false, const_cast<AdjState &>(s)); }
 AdjFields adj_view(const clmi_ctx *c, const AdjState &s)
 {
   AdjFields v{};
@@ -1325,17 +1331,22 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   }
   CHK(clmi_get_background(c, &b_init));
   CHK(clmi_lf_init(c, 0));
-  CHK(clmi_get_background(c, &tape[0]));
+  // the background after every step is copied on the device (DevState starts with it) and read back once
+  static_assert(offsetof(DevState, bg) == 0, "DevState must start with the background");
+  CHK(flush(c));
+  HIPCHK(c, hipMemcpyAsync(run.tape, c->dstate, sizeof(clmi_background), hipMemcpyDeviceToDevice, c->stream));
   for (int n = 0; n < nsteps; ++n) {
     if (n % M == 0) CHK(adj_save(c, run.checkpoints[n / M]));
     CHK(adj_forward_step(c));
-    CHK(clmi_get_background(c, &tape[n + 1]));
+    HIPCHK(c, hipMemcpyAsync(run.tape + n + 1, c->dstate, sizeof(clmi_background), hipMemcpyDeviceToDevice,
+                             c->stream));
   }
+  HIPCHK(c, hipMemcpyAsync(tape.data(), run.tape, sizeof(clmi_background) * (size_t)(nsteps + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   CHK(clmi_lf_sync(c));
   CHK(clmi_get_background(c, &b_fin));
   CHK(adj_save(c, run.final_state));
-  HIPCHK(c, hipMemcpyAsync(run.tape, tape.data(), sizeof(clmi_background) * (size_t)(nsteps + 1),
-                           hipMemcpyHostToDevice, c->stream));
 
   // ---- reverse ----
   AdjArgs a{};

@@ -298,6 +298,15 @@ hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red,
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                                const TiledPlan &p, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// Copies of up to kCopySetMax device arrays in one launch (sizes in 16-byte units).
+constexpr int kCopySetMax = 12;
+struct CopySet {
+  const void *src[kCopySetMax];
+  void *dst[kCopySetMax];
+  size_t n16[kCopySetMax];
+  int count;
+};
+hipError_t launch_copy_set(const CopySet &cs, hipStream_t s);
 int scalar_pass_blocks(uint32_t V);
 hipError_t launch_scalar_pass(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               int blocks, hipStream_t s);
