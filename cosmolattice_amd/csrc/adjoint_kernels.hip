@@ -101,6 +101,7 @@ __device__ __forceinline__ double d2mono(const ScalarPoly &P, int t, const doubl
 //   lam_phi_f += lamG (-2/N^3) L(phi_f) + lamU V'_f(phi) / N^3
 //   lam_pi_f  += lamP (2/N^3) pi_f                        (when lamP != 0 or with_pi)
 //   red[0] = sum lam_phi . pi,  red[1 + t] = sum m_t(phi)
+// (pi from a.fwd_red when it is set, else from a.fwd)
 template <int NF, bool F32, int PK>
 __global__ __launch_bounds__(kBlock) void k_adj_fields(const AdjArgs a, double lamG, double lamU, double lamP,
                                                        int with_pi)
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fields(const AdjArgs a, double l
       double l = a.lam_phi[f][s];
       l += lamG * (-2.0 * invV) * lap + lamU * dv * invV;
       a.lam_phi[f][s] = l;
-      const double p = ld_pi<F32>(a.fwd, f, s);
+      const double p = ld_pi<F32>(a.fwd_red.pi[0] ? a.fwd_red : a.fwd, f, s);
       acc[0] += l * p;
       if (with_pi) a.lam_pi[f][s] += lamP * 2.0 * invV * p;
     }
@@ -164,7 +165,7 @@ template <int NF, bool F32> __global__ __launch_bounds__(kBlock) void k_adj_pi(c
 // FIELDS: then the fields transpose (A) of the same state, with the Laplacian and V' the kick already
 // evaluated, in the order (A) applies them:
 //   lam_phi_f += lamG (-2/N^3) L(phi_f) + lamU V'_f(phi) / N^3
-//   red[F] = sum lam_phi . pi,  red[F + 1 + t] = sum m_t(phi),  F = kAdjFieldSlots
+//   red[F] = sum lam_phi . pi,  red[F + 1 + t] = sum m_t(phi),  F = kAdjFieldSlots  (pi as in (A))
 template <int NF, bool F32, bool FIELDS, int PK>
 __global__ __launch_bounds__(kBlock) void k_adj_kick(const AdjArgs a, double c, double aK, double aV)
 {
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_kick(const AdjArgs a, double c, 
       acc[1] += lp[f] * dv;
       if (FIELDS) {
         l += lamG * (-2.0 * invV) * lapf + lamU * dv * invV;
-        acc[kAdjFieldSlots] += l * ld_pi<F32>(a.fwd, f, s);
+        acc[kAdjFieldSlots] += l * ld_pi<F32>(a.fwd_red.pi[0] ? a.fwd_red : a.fwd, f, s);
       }
       a.lam_phi[f][s] = l;
     }

@@ -1408,33 +1408,43 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   // Per reverse step n: the pi transpose of step n, then one pass with the kick transpose of step n
   // fused with the fields transpose of state n (which the original order runs first thing in step
   // n - 1), each followed by its scalar chain.  The fields transpose of the final state runs alone.
+  //
+  // A segment is recomputed with plain leapfrog steps from its checkpoint (a materialised state
+  // phi_n0, pi_{n0-1/2}), without flushing the deferred drift: after step n the context holds
+  // S_n = (phi_n, pi_{n+1/2}), which is what the step's transposes read -- phi_n for the kick and fields
+  // transposes, pi_{n+1/2} for the pi transpose; the fields transpose's sum lam_phi . pi_{n-1/2} takes
+  // the momentum of S_{n-1} (of the checkpoint for n = n0).  The scalar updates of the fused pass are
+  // the reference's sequence, so S_n equals the forward run's state bit for bit.
   AdjArgs ad = a;
   ad.sc = run.sc;
   for (int k = ncp - 1; k >= 0; --k) {
     const int n0 = k * M, n1 = std::min(nsteps, n0 + M);
-    // recompute the segment's states n0..n1 from checkpoint k
+    // the previous segment left a deferred drift that belongs to states no longer needed: drop it
+    // before the background call below, which would otherwise flush it into the restored fields
+    c->pending_drift = false;
     CHK(adj_restore(c, run.checkpoints[k]));
     CHK(clmi_set_background(c, &tape[n0]));
     c->synced = (n0 == 0);
-    c->pending_drift = false;
     for (int n = n0; n < n1; ++n) {
+      CHK(clmi_lf_evolve(c, 1));
       CHK(adj_save(c, run.segment[n - n0]));
-      CHK(adj_forward_step(c));
     }
-    CHK(adj_save(c, run.segment[n1 - n0]));
     if (k == ncp - 1) {
-      // storeFieldsAverages of the final phi, then the drift's sum lam_phi . pi
-      ad.fwd = adj_view(c, run.segment[n1 - n0]);
+      // storeFieldsAverages of the final phi (unchanged by the sync), then the drift's sum
+      // lam_phi . pi_{T-1/2}
+      ad.fwd = adj_view(c, run.final_state);
+      ad.fwd_red = adj_view(c, run.segment[n1 - 1 - n0]);
       HIPCHK(c, launch_adj_fields(ad, c->f32, 0.0, 0.0, 0.0, false, c->stream));
       HIPCHK(c, launch_adj_chain_fields(run.sc, run.red, 0, run.tape, nsteps - 1, nsteps == 1 ? 0.5 : 1.0, V, mc, nt,
                                         ex, c->stream));
     }
     for (int n = n1 - 1; n >= n0; --n) {
-      // drift and <pi^2> transposes of step n: pi_{n+1/2} is the momentum of state n + 1
-      ad.fwd = adj_view(c, run.segment[n + 1 - n0]);
+      // drift and <pi^2> transposes of step n (pi_{n+1/2})
+      ad.fwd = adj_view(c, run.segment[n - n0]);
+      ad.fwd_red = AdjFields{};
       HIPCHK(c, launch_adj_pi(ad, c->f32, 0.0, 0.0, c->stream));
       // kick of step n on phi_n, with the fields transpose of phi_n for step n - 1
-      ad.fwd = adj_view(c, run.segment[n - n0]);
+      ad.fwd_red = adj_view(c, n > n0 ? run.segment[n - 1 - n0] : run.checkpoints[k]);
       HIPCHK(c, launch_adj_kick_fields(ad, c->f32, n > 0, c->stream));
       HIPCHK(c, launch_adj_chain_kick(run.sc, run.red, run.tape, n, mc, nt, ex, c->stream));
       if (n > 0)
@@ -1442,6 +1452,7 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
                                           mc, nt, ex, c->stream));
     }
   }
+  c->pending_drift = false;
   {
     AdjScalars h{};
     HIPCHK(c, hipMemcpyAsync(&h, run.sc, sizeof(AdjScalars), hipMemcpyDeviceToHost, c->stream));

@@ -223,6 +223,7 @@ struct AdjScalars {
 };
 struct AdjArgs {
   AdjFields fwd;
+  AdjFields fwd_red;  // momenta for the fields transpose's sum lam_phi . pi (unset, pi[0] null: fwd's)
   double *lam_phi[CLMI_MAX_SCALARS];  // FP64 adjoint fields
   double *lam_pi[CLMI_MAX_SCALARS];
   int nf;
