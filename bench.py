@@ -244,10 +244,10 @@ def adjoint_bench(clmi, N=128, kIR=0.05, dt=0.005, steps=2000, check_steps=(500,
     coeffs = [c for c, _ in spec.terms]
     band = (0.5, 1.5)
 
-    def run(prec, cvals, nsteps, p0=phi0, grads=True):
+    def run(prec, cvals, nsteps, p0=phi0, grads=True, q0=pi0):
         c = torch.tensor(cvals, dtype=torch.float64, requires_grad=grads)
         p = torch.from_numpy(p0.copy()).requires_grad_(grads)
-        q = torch.from_numpy(pi0.copy()).requires_grad_(grads)
+        q = torch.from_numpy(q0.copy()).requires_grad_(grads)
         t0 = time.perf_counter()
         phiT, piT, bgT = leapfrog_run(c, p, q, spec, N, dx, dt, nsteps, prec, True, 0)
         L = chi_energy_fraction_and_band(phiT, piT, bgT, c, spec, dx, kIR, band)
@@ -278,9 +278,14 @@ def adjoint_bench(clmi, N=128, kIR=0.05, dt=0.005, steps=2000, check_steps=(500,
     checks = []
     x = np.arange(N)
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    prng = np.random.default_rng(seed + 17)
+    phi0_p = phi0 * (1.0 + 2.0 ** -24 * prng.uniform(-1.0, 1.0, size=phi0.shape))
+    pi0_p = pi0 * (1.0 + 2.0 ** -24 * prng.uniform(-1.0, 1.0, size=pi0.shape))
     for ns in check_steps:
         _, _, gc, gp = run(clmi.CLMI_F64, coeffs, ns)
         _, _, gc_32, gp_32 = run(clmi.CLMI_F32_BULK, coeffs, ns)
+        # chaotic floor: the FP64 adjoint from initial fields perturbed at FP32 rounding level
+        _, _, gc_p, gp_p = run(clmi.CLMI_F64, coeffs, ns, p0=phi0_p, q0=pi0_p)
         # A central difference errs by truncation at large steps and by rounding at small ones; the
         # adjoint is checked against the best of the sequence (the step is reported with it).
         def best(adj, fd):
@@ -306,7 +311,11 @@ def adjoint_bench(clmi, N=128, kIR=0.05, dt=0.005, steps=2000, check_steps=(500,
             em, sm = best(float(mg64[j][0]), fdm)
             fd_modes.append({"mode": [int(v) for v in modes[j]], "adjoint_f64": float(mg64[j][0]),
                              "central_difference_f64_by_rel_step": fdm, "best_rel_err": em, "at_rel_step": sm})
+        mgp = mode_gradients(gp_p[0], modes)
         checks.append({"steps": ns,
+                       "dL_dq_rel_gap_f32_bulk_vs_f64": abs(gc_32[1] - gc[1]) / abs(gc[1]),
+                       "dL_dq_rel_gap_floor_f64_perturbed_rel_2^-24": abs(gc_p[1] - gc[1]) / abs(gc[1]),
+                       "modes_max_rel_diff_floor_f64_perturbed": float(np.max(np.abs(mgp - mg64)) / np.max(np.abs(mg64))),
                        "dL_dq": {"adjoint_f64": 0.5 * gc[1], "adjoint_f32_bulk": 0.5 * gc_32[1],
                                  "central_difference_f64_by_rel_step": fd,
                                  "best_rel_err_f64": e64, "at_rel_step_f64": s64,
@@ -315,9 +324,28 @@ def adjoint_bench(clmi, N=128, kIR=0.05, dt=0.005, steps=2000, check_steps=(500,
                        "modes_max_rel_diff_f32_vs_f64": float(np.max(np.abs(mg32 - mg64)) / np.max(np.abs(mg64)))})
         log(f"[bench] adjoint check {ns} steps: dL/dq best rel err {e64:.2e} at rel step {s64:g}")
     long_fd = {f"{rel:g}": fd_q(steps, rel) for rel in (1e-4, 1e-6, 1e-8)}
+    # the library call alone (no host-side loss, transfers or torch), on a prepared lattice
+    call = {}
+    for name, prec in (("f32_bulk", clmi.CLMI_F32_BULK), ("f64", clmi.CLMI_F64)):
+        lat = clmi.Lattice(spec, N, dx, dt, prec, expansion=True)
+        for f in range(2):
+            lat.upload(FLD_S, f, 0, phi0[f])
+            lat.upload(PI_S, f, 0, pi0[f])
+        seeds = [np.zeros((N, N, N)), np.full((N, N, N), 1e-3)]
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.adjoint_lf(steps, seeds, [np.zeros((N, N, N))] * 2, (0.0, 0.0))
+        call[name] = time.perf_counter() - t0
+        lat.close()
+    log(f"[bench] adjoint call alone: {call}")
     return {"config": f"lphi4 2 fields {N}^3, q=100, kIR={kIR}, dt={dt}, {steps} leapfrog steps, expansion; "
                       f"loss = E_chi/E_tot + chi power in k in [{band[0]}, {band[1]}]",
             "forward_plus_adjoint_seconds": {"f32_bulk": t32, "f64": t64},
+            "forward_plus_adjoint_seconds_note": "torch round trip: two lattice set-ups, host transfers, the loss "
+                                                 "(CPU FFT) and its backward, plus clmi_adjoint_lf",
+            "adjoint_call_seconds": call,
+            "adjoint_call_note": "clmi_adjoint_lf alone on a prepared lattice: forward with checkpoints, per-segment "
+                                 "recomputation, reverse sweep (host FP64 seeds in, gradients out)",
             "loss": {"f32_bulk": L32, "f64": L64},
             "dL_dq": {"f32_bulk": 0.5 * gc32[1], "f64": 0.5 * gc64[1]},
             "gradient_checks": checks,
@@ -375,6 +403,7 @@ def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000, every=250):
         _, _, rhs = lat.hubble_constraint()
         lat.set_background(aDotI=math.sqrt(rhs))
         lat.lf_init()
+        tune = lat.autotune(3)  # x-chunk of the charged-scalar + U(1) pass; results do not depend on it
         sec, kms, launches = 0.0, 0.0, 0
         fried, gauss = [], []
         done = 0
@@ -392,7 +421,9 @@ def u1_bench(clmi, steps, N=256, kIR=0.05, dt=0.005, seed=9000, every=250):
                      "hbm_frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS,
                      "max_abs_friedmann": float(np.max(np.abs(fried))), "end_friedmann": float(fried[-1]),
                      "max_gauss_violation": float(np.max(gauss)), "end_gauss_violation": float(gauss[-1]),
-                     "aI_end": recs[name][2]}
+                     "aI_end": recs[name][2],
+                     "kernel": next((ln[len("# kernel "):] for ln in tune.splitlines() if ln.startswith("# kernel ")),
+                                    "generic")}
         log(f"[bench] u1 {name}: {out[name]['ms_per_step']:.4f} ms/step, frac {out[name]['hbm_frac']:.3f}, "
             f"max |Friedmann| {out[name]['max_abs_friedmann']:.3e}, max Gauss {out[name]['max_gauss_violation']:.3e}")
         lat.close()

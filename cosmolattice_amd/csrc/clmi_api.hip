@@ -47,6 +47,7 @@ struct clmi_ctx {
   double *partials = nullptr;
   double *dscratch = nullptr;  // small device scratch for row sums
   int blocks_s = 0, blocks_g = 0;
+  int rows_g = 0;  // partial-sum rows a reducing charged-scalar + U(1) pass writes
   bool tiled = false;  // scalar passes use the tiled kernel with plan tplan
   TiledPlan tplan;
   bool rowk = false;   // scalar passes use the full-row kernel with plan rplan
@@ -209,6 +210,44 @@ ScalarPassArgs scalar_args(const clmi_ctx *c, int fi, int fo, int pi, int po, do
   return a;
 }
 
+// Arguments of the charged-scalar + U(1) pass from buffers fi/fo (fields) and pi/po (momenta).
+GaugePassArgs gauge_args(const clmi_ctx *c, int fi, int fo, int pi, int po, double cdt_s, double ckick_g, int row0)
+{
+  GaugePassArgs a{};
+  for (int k = 0; k < 2; ++k) {
+    a.phi_in[k] = c->fCS[k][fi].p;
+    a.philo_in[k] = c->fCS[k][fi].lo;
+    a.pi_in[k] = c->pCS[k][pi].p;
+    a.phi_out[k] = c->fCS[k][fo].p;
+    a.philo_out[k] = c->fCS[k][fo].lo;
+    a.pi_out[k] = c->pCS[k][po].p;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (c->cfg.n_u1 > 0) {
+      a.A_in[d] = c->fU[d][fi].p;
+      a.Alo_in[d] = c->fU[d][fi].lo;
+      a.piA_in[d] = c->pU[d][pi].p;
+      a.A_out[d] = c->fU[d][fo].p;
+      a.Alo_out[d] = c->fU[d][fo].lo;
+      a.piA_out[d] = c->pU[d][po].p;
+    }
+  a.N = c->N;
+  a.V = c->V;
+  a.has_u1 = c->cfg.n_u1 > 0;
+  a.dx = c->cfg.dx;
+  a.inv_dx = 1.0 / c->cfg.dx;
+  a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
+  a.gQ = c->cfg.gQ;
+  const double fw = c->cfg.fStar / c->cfg.omegaStar;
+  a.jnorm = 2.0 / c->cfg.dx * (fw * fw);
+  a.cdt = cdt_s;
+  a.ckick = ckick_g;
+  a.st = c->dstate;
+  a.partials = c->partials + (size_t)row0 * kNumRed;
+  a.poly = c->cpoly;
+  return a;
+}
+
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
 int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
 {
@@ -236,47 +275,16 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
     c->cur_pS = po;
   }
   if (c->cfg.n_cscalars > 0) {
-    GaugePassArgs a{};
     const int fi = c->cur_fG, fo = drift ? 1 - c->cur_fG : c->cur_fG;
     const int pi = c->cur_pG, po = (drift && kick) ? 1 - c->cur_pG : c->cur_pG;
-    for (int k = 0; k < 2; ++k) {
-      a.phi_in[k] = c->fCS[k][fi].p;
-      a.philo_in[k] = c->fCS[k][fi].lo;
-      a.pi_in[k] = c->pCS[k][pi].p;
-      a.phi_out[k] = c->fCS[k][fo].p;
-      a.philo_out[k] = c->fCS[k][fo].lo;
-      a.pi_out[k] = c->pCS[k][po].p;
-    }
-    for (int d = 0; d < 3; ++d)
-      if (c->cfg.n_u1 > 0) {
-        a.A_in[d] = c->fU[d][fi].p;
-        a.Alo_in[d] = c->fU[d][fi].lo;
-        a.piA_in[d] = c->pU[d][pi].p;
-        a.A_out[d] = c->fU[d][fo].p;
-        a.Alo_out[d] = c->fU[d][fo].lo;
-        a.piA_out[d] = c->pU[d][po].p;
-      }
-    a.N = c->N;
-    a.V = c->V;
-    a.has_u1 = c->cfg.n_u1 > 0;
-    a.dx = c->cfg.dx;
-    a.inv_dx = 1.0 / c->cfg.dx;
-    a.inv_dx2 = 1.0 / (c->cfg.dx * c->cfg.dx);
-    a.gQ = c->cfg.gQ;
-    const double fw = c->cfg.fStar / c->cfg.omegaStar;
-    a.jnorm = 2.0 / c->cfg.dx * (fw * fw);
-    a.cdt = cdt_s;
-    a.ckick = ckick_g;
-    a.st = c->dstate;
-    a.partials = c->partials + (size_t)r * kNumRed;
-    a.poly = c->cpoly;
-    if (red && (size_t)r + c->blocks_g > c->partial_rows)
+    const GaugePassArgs a = gauge_args(c, fi, fo, pi, po, cdt_s, ckick_g, r);
+    if (red && (size_t)r + c->rows_g > c->partial_rows)
       return fail(c, CLMI_E_INVALID, "partial-sum buffer too small for the charged-scalar pass");
     if (c->gtiled)
       HIPCHK(c, launch_gauge_tiled(c->f32, c->gTY, drift, kick, red, a, c->blocks_g, c->gXC, c->stream));
     else
       HIPCHK(c, launch_gauge_pass(c->f32, drift, kick, red, a, c->blocks_g, c->stream));
-    if (red) r += c->blocks_g;
+    if (red) r += c->rows_g;
     c->cur_fG = fo;
     c->cur_pG = po;
   }
@@ -458,6 +466,7 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
 
   c->blocks_s = scalar_pass_blocks(c->V);
   c->blocks_g = gauge_pass_blocks(c->V);
+  c->rows_g = c->blocks_g;
   if (!(g.flags & CLMI_FLAG_NO_TILED)) {
     // Default x-chunk of the full-row pass: the longest of 32, 16, 8 that still gives every CU a
     // workgroup, else the shortest (small lattices).  clmi_autotune refines it; results do not depend on it.
@@ -491,13 +500,14 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
           c->tplan = p;
           break;
         }
-    int gxc = 0, gblocks = 0;
+    int gxc = 0, gblocks = 0, grows_t = 0;
     const int gty = (g.flags & CLMI_FLAG_GAUGE_TY8) ? 8 : 4;
-    if (g.n_cscalars > 0 && gauge_tiled_plan(g.N, gty, kMaxPassBlocks, &gxc, &gblocks)) {
+    if (g.n_cscalars > 0 && gauge_tiled_plan(g.N, gty, ncu, &gxc, &gblocks, &grows_t)) {
       c->gtiled = true;
       c->gTY = gty;
       c->gXC = gxc;
       c->blocks_g = gblocks;
+      c->rows_g = grows_t;
     }
   }
   // Partial-sum rows: a reducing step pass writes the scalar sector's rows, then the charged-scalar + U(1)
@@ -506,7 +516,7 @@ int clmi_create(const clmi_config *cfg, clmi_ctx **out)
                        : c->rowk        ? (size_t)c->rplan.rows
                        : c->tiled       ? (size_t)c->tplan.rows
                                         : (size_t)c->blocks_s;
-  const size_t grows = g.n_cscalars > 0 ? (size_t)c->blocks_g : 0;
+  const size_t grows = g.n_cscalars > 0 ? (size_t)c->rows_g : 0;
   const size_t rows = std::max(2 * (size_t)kMaxPassBlocks, srows + grows) + 16;
   c->partial_rows = rows;
   if (hipMalloc(&c->partials, rows * kNumRed * sizeof(double)) != hipSuccess ||
@@ -853,8 +863,9 @@ int clmi_gauss_u1(clmi_ctx *c, double out[3])
   const double fw = c->cfg.fStar / c->cfg.omegaStar;
   a.qnorm_gQ = 2.0 / c->cfg.dx * (fw * fw) * c->cfg.gQ;
   a.partials = c->partials;
-  HIPCHK(c, launch_gauss(a, c->blocks_g, c->stream));
-  HIPCHK(c, launch_sum_rows(c->partials, c->blocks_g, c->dscratch, c->stream));
+  const int gblocks = gauge_pass_blocks(c->V);
+  HIPCHK(c, launch_gauss(a, gblocks, c->stream));
+  HIPCHK(c, launch_sum_rows(c->partials, gblocks, c->dscratch, c->stream));
   double h[kNumRed];
   HIPCHK(c, hipMemcpyAsync(h, c->dscratch, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1861,12 +1872,67 @@ int clmi_profile_enable(clmi_ctx *c, int on)
   return CLMI_OK;
 }
 
+namespace {
+// x-chunk of the tiled charged-scalar + U(1) pass: fused passes from the current buffers into the spare
+// ones (no parity flip, no background update, so the context state is unchanged) for every valid XC;
+// the partial rows do not depend on XC, so neither do the results.
+int autotune_gauge(clmi_ctx *c, int reps, std::string &rep)
+{
+  const GaugePassArgs a =
+      gauge_args(c, c->cur_fG, 1 - c->cur_fG, c->cur_pG, 1 - c->cur_pG, c->cfg.dt, 0.0, 0);
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  double best = 1e300;
+  int best_xc = c->gXC, best_blocks = c->blocks_g;
+  std::string lines;
+  hipError_t e = hipSuccess;
+  for (int xc = 8; xc <= c->N && e == hipSuccess; xc *= 2) {
+    int blocks = 0, rows = 0;
+    if (!gauge_tiled_shape(c->N, c->gTY, xc, &blocks, &rows) || rows != c->rows_g) continue;
+    auto launch = [&]() { return launch_gauge_tiled(c->f32, c->gTY, true, true, true, a, blocks, xc, c->stream); };
+    e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    const double us = 1e3 * ms / reps;
+    lines += "gauge XC=" + std::to_string(xc) + " " + std::to_string(us) + "\n";
+    if (us < best) {
+      best = us;
+      best_xc = xc;
+      best_blocks = blocks;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(c, CLMI_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  c->gXC = best_xc;
+  c->blocks_g = best_blocks;
+  rep += "# kernel k_gauge_tiled TY=" + std::to_string(c->gTY) + " XC=" + std::to_string(best_xc) + "\n" + lines;
+  return CLMI_OK;
+}
+}  // namespace
+
 int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
 {
   CHK(check_ctx(c));
   if (report && report_len) report[0] = '\0';
-  if ((!c->tiled && !c->rowk) || c->cfg.n_scalars == 0) return CLMI_OK;
   if (reps < 1) reps = 1;
+  if ((!c->tiled && !c->rowk) || c->cfg.n_scalars == 0) {
+    if (!c->gtiled) return CLMI_OK;
+    CHK(flush(c));
+    std::string rep;
+    CHK(autotune_gauge(c, reps, rep));
+    if (report && report_len) {
+      std::strncpy(report, rep.c_str(), report_len - 1);
+      report[report_len - 1] = '\0';
+    }
+    return CLMI_OK;
+  }
   CHK(flush(c));
   // fused drift+kick+reduce passes from the current buffers into the spare ones; parities are not
   // flipped and no background update runs, so the context state is unchanged
@@ -1930,6 +1996,7 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
     rep = "# kernel k_scalar_tiled XC=" + std::to_string(best_plan.XC) + " NT=" + std::to_string((int)best_plan.NT) +
           " PK=" + std::to_string(poly_kind(c->spoly, c->cfg.n_scalars)) + "\n" + rep;
   }
+  if (c->gtiled) CHK(autotune_gauge(c, reps, rep));
   if (report && report_len) {
     std::strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = '\0';

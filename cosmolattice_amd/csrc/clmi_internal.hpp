@@ -176,7 +176,8 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *row)
 }
 
 // ---- tiled charged scalar + U(1) pass (gauge_tiled.hip) ----
-bool gauge_tiled_plan(int N, int ty, int max_blocks, int *XC, int *blocks);
+bool gauge_tiled_shape(int N, int ty, int XC, int *blocks, int *rows);
+bool gauge_tiled_plan(int N, int ty, int ncu, int *XC, int *blocks, int *rows);
 hipError_t launch_gauge_tiled(bool f32, int ty, bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks,
                               int XC, hipStream_t s);
 

@@ -300,8 +300,9 @@ int clmi_adjoint_lf(clmi_ctx *ctx, int nsteps, int checkpoint_every, int bg_from
  * candidate plan of the tiled lattice pass -- x-chunk length and store mode -- with `reps` fused
  * passes on the context's own data, without changing the context's fields or background, keeps
  * the fastest and writes one "XC=<n> NT=<0|1> <microseconds>" line per candidate into report
- * (if non-NULL).  All candidates give bit-identical results, including the volume sums.  A no-op
- * returning CLMI_OK where the tiled pass does not apply. */
+ * (if non-NULL).  With a charged scalar + U(1) it also times the x-chunk of that sector's tiled
+ * pass ("gauge XC=<n> <microseconds>" lines).  All candidates give bit-identical results, including
+ * the volume sums.  A no-op returning CLMI_OK where no tiled pass applies. */
 int clmi_autotune(clmi_ctx *ctx, int reps, char *report, size_t report_len);
 
 /* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */

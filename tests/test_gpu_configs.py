@@ -123,6 +123,35 @@ def test_c3_u1_256_tiled_bit_identical_to_generic(gpu, prec):
         assert np.array_equal(res[0][kc], res[1][kc]), kc
 
 
+@pytest.mark.parametrize("prec", [CLMI_F64, CLMI_F32_BULK])
+def test_c3_u1_256_autotune_keeps_state_and_results(gpu, prec):
+    """At 256^3 the autotuner times every x-chunk of the tiled charged-scalar + U(1) pass on the live
+    lattice without changing it, and the chunk it picks gives fields, momenta and the background (volume
+    sums included) bit-identical to the default chunk over 6 leapfrog steps with expansion: the pass
+    writes one partial row per group of 8 planes whatever the chunk."""
+    say(f"case c3 autotune prec={prec}")
+    outs = []
+    for tune in (False, True):
+        _, lat, _ = u1_vacuum(prec)
+        lat.lf_init()
+        lat.lf_evolve(3)
+        if tune:
+            before = {kc: lat.download(kc[0], 0, kc[1]) for kc in U1_KINDS}
+            bg_before = lat.background()
+            rep = lat.autotune(2)
+            say(rep)
+            assert "gauge XC=" in rep
+            assert all(np.array_equal(before[kc], lat.download(kc[0], 0, kc[1])) for kc in U1_KINDS)
+            assert lat.background() == bg_before
+        lat.lf_evolve(3)
+        lat.lf_sync()
+        outs.append(({kc: lat.download(kc[0], 0, kc[1]) for kc in U1_KINDS}, lat.background()))
+        lat.close()
+    for kc in U1_KINDS:
+        assert np.array_equal(outs[0][0][kc], outs[1][0][kc]), kc
+    assert outs[0][1] == outs[1][1]
+
+
 def test_c3_u1_256_fp64_matches_oracle(gpu, oracle_mod):
     """lphi4U1 at 256^3 with expansion: 5 FP64 leapfrog steps plus a sync on the GPU equal the oracle's
     from the same initial arrays and background, within 1e-10 on all ten arrays, the background and the
