@@ -66,6 +66,19 @@ template <> struct GRaw<true> {
   }
 };
 
+// Outputs are written with non-temporal stores: nothing in the pass reads them back, and keeping them
+// out of L2 leaves room for the halo rows that neighbouring tiles re-read (256^3: 663 -> 643 us FP64,
+// 474 -> 461 us FP32-bulk).
+template <bool F32> __device__ __forceinline__ void nt_store_v(void *f, float *lo, uint32_t i, typename GVal<F32>::V v)
+{
+  if constexpr (F32) {
+    __builtin_nontemporal_store(v.h, (float *)f + i);
+    __builtin_nontemporal_store(v.l, lo + i);
+  } else {
+    __builtin_nontemporal_store(v, (double *)f + i);
+  }
+}
+
 template <bool F32, bool DRIFT, bool KICK, bool RED, int GTY>
 __global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a, const int XC)
 {
@@ -192,9 +205,9 @@ __global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a,
 
     const uint32_t s = site(x, y, z);
     if (DRIFT) {
-      for (int q = 0; q < 2; ++q) G::store(a.phi_out[q], a.philo_out[q], s, g.fc[q]);
+      for (int q = 0; q < 2; ++q) nt_store_v<F32>(a.phi_out[q], a.philo_out[q], s, g.fc[q]);
       if (u1)
-        for (int d = 0; d < 3; ++d) G::store(a.A_out[d], a.Alo_out[d], s, g.Ac[d]);
+        for (int d = 0; d < 3; ++d) nt_store_v<F32>(a.A_out[d], a.Alo_out[d], s, g.Ac[d]);
     }
     P pin[2], pAin[3] = {0, 0, 0}, pout[2], pAout[3];
     if (KICK) {
@@ -211,9 +224,9 @@ __global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a,
     else
       gauge_site_f64<KICK, RED>(k, a.poly, g, pin, pAin, pout, pAout, acc);
     if (KICK) {
-      for (int q = 0; q < 2; ++q) ((P *)a.pi_out[q])[s] = pout[q];
+      for (int q = 0; q < 2; ++q) __builtin_nontemporal_store(pout[q], (P *)a.pi_out[q] + s);
       if (u1)
-        for (int d = 0; d < 3; ++d) ((P *)a.piA_out[d])[s] = pAout[d];
+        for (int d = 0; d < 3; ++d) __builtin_nontemporal_store(pAout[d], (P *)a.piA_out[d] + s);
     }
     if (RED && (i + 1) % kGaugeG == 0) {
       double v[kNumRed] = {0, 0, acc[2], acc[0], acc[1], acc[3], acc[4], 0};
