@@ -11,7 +11,7 @@
 namespace clmi {
 namespace {
 
-constexpr int kUpdThreads = 256;
+constexpr int kUpdThreads = 512;  // 8 waves: two per SIMD, so the updating thread keeps its state in registers
 
 __global__ __launch_bounds__(kUpdThreads) void k_update(DevState *st, const double *partials, int rows,
                                                         ModelConsts mc, UpdateOps ops)

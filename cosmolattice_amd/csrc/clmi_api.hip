@@ -48,6 +48,7 @@ struct clmi_ctx {
   double *dscratch = nullptr;  // small device scratch for row sums
   int blocks_s = 0, blocks_g = 0;
   int rows_g = 0;  // partial-sum rows a reducing charged-scalar + U(1) pass writes
+  clmi_background *snap = nullptr;  // device target of the next storeFieldsAverages snapshot (adjoint tape)
   bool tiled = false;  // scalar passes use the tiled kernel with plan tplan
   TiledPlan tplan;
   bool rowk = false;   // scalar passes use the full-row kernel with plan rplan
@@ -310,13 +311,22 @@ int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> 
   return CLMI_OK;
 }
 
-// A lattice pass followed by the background update `ops` (skipped when do_update is false).
+// A lattice pass followed by the background update `ops` (skipped when do_update is false).  When the
+// context holds a snapshot target (c->snap) and the update starts with storeFieldsAverages, the
+// background after that operation -- the state a flush at this point would leave -- is written there.
 int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g,
                 bool do_update, std::initializer_list<std::pair<int, double>> ops)
 {
   int rows = 0;
   CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows));
-  if (do_update) HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, make_ops(ops), c->stream));
+  if (do_update) {
+    UpdateOps u = make_ops(ops);
+    if (c->snap && u.n > 0 && u.op[0] == OP_STORE_FIELDS) {
+      u.snap = c->snap;
+      c->snap = nullptr;
+    }
+    HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
+  }
   return CLMI_OK;
 }
 
@@ -1282,15 +1292,6 @@ int adj_read_red(clmi_ctx *c, const AdjRun &r, double out[kAdjRed])
   return CLMI_OK;
 }
 
-// One forward leapfrog step exactly as clmi_lf_evolve runs it, then the deferred drift, so the
-// context holds a materialised state again.
-int adj_forward_step(clmi_ctx *c)
-{
-  CHK(clmi_lf_evolve(c, 1));
-  CHK(flush(c));
-  return CLMI_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -1342,19 +1343,35 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   }
   CHK(clmi_get_background(c, &b_init));
   CHK(clmi_lf_init(c, 0));
-  // the background after every step is copied on the device (DevState starts with it) and read back once
+  // Plain fused leapfrog steps, flushed only where a checkpoint needs the materialised state.  The tape
+  // entry of step n is the background after storeFieldsAverages of step n - 1 (what a flush leaves);
+  // the update that runs that operation writes it on the device (c->snap), and the tape is read back
+  // once.  Without expansion no update runs and the background stays that of the start.
   static_assert(offsetof(DevState, bg) == 0, "DevState must start with the background");
   CHK(flush(c));
   HIPCHK(c, hipMemcpyAsync(run.tape, c->dstate, sizeof(clmi_background), hipMemcpyDeviceToDevice, c->stream));
   for (int n = 0; n < nsteps; ++n) {
-    if (n % M == 0) CHK(adj_save(c, run.checkpoints[n / M]));
-    CHK(adj_forward_step(c));
-    HIPCHK(c, hipMemcpyAsync(run.tape + n + 1, c->dstate, sizeof(clmi_background), hipMemcpyDeviceToDevice,
-                             c->stream));
+    if (n % M == 0) {
+      if (c->pending_drift) {
+        c->snap = run.tape + n;
+        CHK(flush(c));
+      }
+      CHK(adj_save(c, run.checkpoints[n / M]));
+    }
+    if (c->pending_drift) c->snap = run.tape + n;
+    CHK(clmi_lf_evolve(c, 1));
   }
+  c->snap = run.tape + nsteps;
+  CHK(flush(c));
+  c->snap = nullptr;
   HIPCHK(c, hipMemcpyAsync(tape.data(), run.tape, sizeof(clmi_background) * (size_t)(nsteps + 1),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!ex) {
+    for (int n = 1; n <= nsteps; ++n) tape[n] = tape[0];
+    HIPCHK(c, hipMemcpyAsync(run.tape, tape.data(), sizeof(clmi_background) * (size_t)(nsteps + 1),
+                             hipMemcpyHostToDevice, c->stream));
+  }
   CHK(clmi_lf_sync(c));
   CHK(clmi_get_background(c, &b_fin));
   CHK(adj_save(c, run.final_state));

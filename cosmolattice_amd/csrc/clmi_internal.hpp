@@ -93,6 +93,7 @@ struct UpdateOps {
   int op[6];
   int pad;
   double w[6];
+  clmi_background *snap;  // non-null: the background after op 0 is also written here
 };
 
 // One real lattice array in either precision.

@@ -58,32 +58,44 @@ __device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &
   return sf_kernel_of(b.aI, b.pi2AvI, b.grad2AvI, b.CSpi2AvI, b.CSgrad2AvI, b.U1pi2AvI, b.U1Mag2AvI, b.potAvI, mc);
 }
 
-// Fixed-order sum of `rows` partial rows by a block of NT threads: thread t takes rows t, t + NT,
-// ... (four loads in flight per slot), then wavefront shuffles, then the wavefront sums in order.
-// Result in out[0..kNumRed) (shared memory); ends with a barrier.
+// Fixed-order sum of `rows` partial rows by a block of NT threads: thread t adds rows t, t + NT, ... in
+// that order, eight rows in flight per iteration (each row one 64-byte line, read as four 16-byte
+// loads), then wavefront shuffles, then the wavefront sums in order.  Result in out[0..kNumRed)
+// (shared memory); ends with a barrier.  The rows are 64-byte aligned (kNumRed = 8 doubles).
 template <int NT>
 __device__ inline void sum_partials(const double *partials, int rows, double *out)
 {
+  static_assert(kNumRed == 8, "a partial row is four 16-byte vectors");
+  typedef double d2 __attribute__((ext_vector_type(2)));
   __shared__ double sh[NT / 64][kNumRed];
   const int t = threadIdx.x;
+  const d2 *P = (const d2 *)partials;
   double v[kNumRed];
 #pragma unroll
   for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
   int r = t;
-  for (; r + 3 * NT < rows; r += 4 * NT) {
-    double q[4][kNumRed];
+  constexpr int U = 8;
+  for (; r + (U - 1) * NT < rows; r += U * NT) {
+    d2 q[U][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int k = 0; k < kNumRed; ++k) q[u][k] = partials[(size_t)(r + u * NT) * kNumRed + k];
+      for (int k = 0; k < 4; ++k) q[u][k] = P[(size_t)(r + u * NT) * 4 + k];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int k = 0; k < kNumRed; ++k) v[k] += q[u][k];
+      for (int k = 0; k < 4; ++k) {
+        v[2 * k] += q[u][k].x;
+        v[2 * k + 1] += q[u][k].y;
+      }
   }
   for (; r < rows; r += NT)
 #pragma unroll
-    for (int k = 0; k < kNumRed; ++k) v[k] += partials[(size_t)r * kNumRed + k];
+    for (int k = 0; k < 4; ++k) {
+      const d2 q = P[(size_t)r * 4 + k];
+      v[2 * k] += q.x;
+      v[2 * k + 1] += q.y;
+    }
 #pragma unroll
   for (int k = 0; k < kNumRed; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
@@ -222,6 +234,7 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
       default:
         break;
     }
+    if (o == 0 && ops.snap) *ops.snap = b;
   }
   st->aK = rpow(b.aI, 1 + mc.alpha);
   st->aV = rpow(b.aI, 3 + mc.alpha);

@@ -252,7 +252,8 @@ def test_c5_adjoint_128_500_steps_matches_central_differences(gpu):
     """Reverse-mode leapfrog on two-field lphi4 at 128^3 (q = 100, kIR = 0.05, dt = 0.005, expansion) over
     500 steps; loss = final energy fraction of chi + chi power in k in [0.5, 1.5].  dL/dq and the
     derivatives by the real parts of two initial phi modes from the FP64 adjoint equal FP64 central
-    differences (relative steps 1e-5 for q, 1e-6 for the modes) within 1e-6."""
+    differences (relative steps 1e-5 for q, 1e-6 for the modes) within 1e-6, and dL/dq with the
+    FP32-bulk forward within 1e-4."""
     import torch
 
     from cosmolattice_amd.adjoint import chi_energy_fraction_and_band, leapfrog_run, mode_gradients
@@ -286,6 +287,14 @@ def test_c5_adjoint_128_500_steps_matches_central_differences(gpu):
     fd_q = 0.5 * (run(cp, phi0, False) - run(cm, phi0, False)) / (2 * h)
     say("dL/dq adjoint", 0.5 * gc[1], "central difference", fd_q)
     assert abs(0.5 * gc[1] - fd_q) <= 1e-6 * abs(fd_q)
+    # the FP32-bulk forward with the FP64 adjoint meets the 1e-4 bar against the same differences
+    c32 = torch.tensor(coeffs, dtype=torch.float64, requires_grad=True)
+    p32 = torch.from_numpy(phi0.copy()).requires_grad_(True)
+    q32 = torch.from_numpy(pi0.copy()).requires_grad_(True)
+    phiT, piT, bgT = leapfrog_run(c32, p32, q32, spec, N, dx, dt, steps, CLMI_F32_BULK, True, 0)
+    chi_energy_fraction_and_band(phiT, piT, bgT, c32, spec, dx, kIR, (0.5, 1.5)).backward()
+    say("dL/dq FP32-bulk adjoint", 0.5 * float(c32.grad[1]))
+    assert abs(0.5 * float(c32.grad[1]) - fd_q) <= 1e-4 * abs(fd_q)
     modes = [np.array([-9, 7, 20]), np.array([-4, 14, 5])]
     mg = mode_gradients(gp[0], modes)
     x = np.arange(N)
