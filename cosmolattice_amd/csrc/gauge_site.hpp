@@ -68,6 +68,16 @@ __device__ __forceinline__ double cs_pot_d(const CsPoly &P, double r2)
   }
   return v;
 }
+__device__ __forceinline__ float cs_pot_f(const CsPoly &P, float r2)
+{
+  float v = 0.0f;
+  for (int k = 0; k < P.nterms; ++k) {
+    float p = (float)P.c[k];
+    for (int i = 0; i < P.h[k]; ++i) p *= r2;
+    v += p;
+  }
+  return v;
+}
 __device__ __forceinline__ float cs_dnorm_over_norm_f(const CsPoly &P, float r2)
 {
   float d = 0.0f;
@@ -194,7 +204,8 @@ __device__ __forceinline__ void link_m1(const GaugeK &k, P2 Ad, float &cm1, floa
   cm1 = -2.0f * sh * sh;
 }
 
-// FP32-bulk site update (stencil differences on hi and lo separately, kick in FP32, sums in FP64).
+// FP32-bulk site update (stencil differences on hi and lo separately, kick and per-site energy terms in
+// FP32, accumulated over sites in FP64, as the scalar FP32-bulk pass does).
 template <bool KICK, bool RED>
 __device__ __forceinline__ void gauge_site_f32(const GaugeK &k, const CsPoly &poly, const GaugeIn<P2> &g,
                                                const float pin[2], const float pAin[3], float pout[2],
@@ -203,7 +214,7 @@ __device__ __forceinline__ void gauge_site_f32(const GaugeK &k, const CsPoly &po
   const P2 fr = g.fc[0], fi = g.fc[1];
   const float pr = fr.h + fr.l, pim = fi.h + fi.l;
   float lap_r = 0.0f, lap_i = 0.0f;
-  double g2 = 0.0;
+  float g2 = 0.0f;  // per-site terms in FP32, accumulated over sites in FP64
   float cm[3], sl[3];
   for (int d = 0; d < 3; ++d) {
     link_m1(k, g.Ac[d], cm[d], sl[d]);
@@ -223,12 +234,8 @@ __device__ __forceinline__ void gauge_site_f32(const GaugeK &k, const CsPoly &po
       lap_r += dr + er;
       lap_i += di + ei;
     }
-    if (RED) {
-      const double gr = (double)dr * k.inv_dx, gi = (double)di * k.inv_dx;
-      g2 += gr * gr + gi * gi;
-    }
+    if (RED) g2 += dr * dr + di * di;
   }
-  const double r2d = ((double)fr.h + fr.l) * ((double)fr.h + fr.l) + ((double)fi.h + fi.l) * ((double)fi.h + fi.l);
   float p_r = pin[0], p_i = pin[1];
   if (KICK) {
     const float nd = cs_dnorm_over_norm_f(poly, pr * pr + pim * pim);
@@ -238,12 +245,12 @@ __device__ __forceinline__ void gauge_site_f32(const GaugeK &k, const CsPoly &po
   pout[0] = p_r;
   pout[1] = p_i;
   if (RED) {
-    acc[0] += (double)p_r * p_r + (double)p_i * p_i;
-    acc[1] += g2;
-    acc[2] += cs_pot_d(poly, r2d);
+    acc[0] += (double)(p_r * p_r + p_i * p_i);
+    acc[1] += (double)g2 * k.inv_dx2;
+    acc[2] += (double)cs_pot_f(poly, pr * pr + pim * pim);
   }
   if (k.u1) {
-    double e2 = 0.0;
+    float e2 = 0.0f;
     for (int i = 0; i < 3; ++i) {
       float pa = pAin[i];
       if (KICK) {
@@ -263,17 +270,17 @@ __device__ __forceinline__ void gauge_site_f32(const GaugeK &k, const CsPoly &po
         pa += k.kG * (lap - gd) - k.kJ * J;
       }
       pAout[i] = pa;
-      e2 += (double)pa * pa;
+      e2 += pa * pa;
     }
     if (RED) {
-      double b2 = 0.0;
+      float b2 = 0.0f;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < i; ++j) {
-          const double f = ((double)p2diff(g.Ap[i][j], g.Ac[j]) - (double)p2diff(g.Ap[j][i], g.Ac[i])) * k.inv_dx;
+          const float f = p2diff(g.Ap[i][j], g.Ac[j]) - p2diff(g.Ap[j][i], g.Ac[i]);
           b2 += f * f;
         }
-      acc[3] += e2;
-      acc[4] += b2;
+      acc[3] += (double)e2;
+      acc[4] += (double)b2 * k.inv_dx2;
     }
   }
 }

@@ -9,15 +9,18 @@
 // values of plane x+2 are loaded while plane x is computed; the momenta of a thread's own site stay
 // in registers from that load until the kick uses them.  The per-site arithmetic is
 // gauge_site.hpp's, so fields are bit-identical to the one-site-per-thread pass.  Volume sums are
-// written per (group of kGaugeG planes, tile), so they do not depend on the x-chunk XC: every XC gives
-// the same partial rows in the same order, and the autotuner may pick any of them.
+// written per (group of G planes, tile), so they do not depend on the x-chunk XC: every XC gives the
+// same partial rows in the same order, and the autotuner may pick any of them.
+#include <algorithm>
+
 #include "gauge_site.hpp"
 
 namespace clmi {
 namespace {
 
 constexpr int GTZ = 64;  // z sites per tile (one per lane)
-constexpr int kGaugeG = 8;  // planes per partial-sum row
+// planes per partial-sum row: N/8 between 8 and 32 (2048 rows at 256^3 for the background update)
+inline int gauge_rows_g(int N) { return std::max(8, std::min(32, N / 8)); }
 constexpr int GNC = 5;   // components: phi_0, phi_1, A_x, A_y, A_z
 
 // Raw (not yet drifted) values of one site's five components.
@@ -80,7 +83,7 @@ template <bool F32> __device__ __forceinline__ void nt_store_v(void *f, float *l
 }
 
 template <bool F32, bool DRIFT, bool KICK, bool RED, int GTY>
-__global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a, const int XC)
+__global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a, const int XC, const int RG)
 {
   using G = GVal<F32>;
   using V = typename G::V;
@@ -228,9 +231,9 @@ __global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a,
       if (u1)
         for (int d = 0; d < 3; ++d) __builtin_nontemporal_store(pAout[d], (P *)a.piA_out[d] + s);
     }
-    if (RED && (i + 1) % kGaugeG == 0) {
+    if (RED && (i + 1) % RG == 0) {
       double v[kNumRed] = {0, 0, acc[2], acc[0], acc[1], acc[3], acc[4], 0};
-      const size_t row = ((size_t)(x / kGaugeG) * nTy + ty) * nTz + tz;
+      const size_t row = ((size_t)(x / RG) * nTy + ty) * nTz + tz;
       block_reduce_store_nt<kNumRed, 64 * GTY>(v, a.partials + row * kNumRed);
       for (int k2 = 0; k2 < 5; ++k2) acc[k2] = 0.0;
     }
@@ -250,10 +253,11 @@ __global__ __launch_bounds__(64 * GTY) void k_gauge_tiled(const GaugePassArgs a,
 template <bool F32, int GTY>
 hipError_t dispatch_tiled(bool drift, bool kick, bool red, const GaugePassArgs &a, int blocks, int XC, hipStream_t s)
 {
+  const int G = gauge_rows_g((int)a.N);
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
 #define CLMI_GT(D, K, R)                                                                              \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                    \
-    hipLaunchKernelGGL((k_gauge_tiled<F32, D, K, R, GTY>), dim3(blocks), dim3(64 * GTY), 0, s, a, XC); \
+    hipLaunchKernelGGL((k_gauge_tiled<F32, D, K, R, GTY>), dim3(blocks), dim3(64 * GTY), 0, s, a, XC, G); \
     break;
   switch (code) {
     CLMI_GT(true, true, true)
@@ -272,15 +276,16 @@ hipError_t dispatch_tiled(bool drift, bool kick, bool red, const GaugePassArgs &
 
 }  // namespace
 
-// Shape of the tiled pass with TY-row tiles (4 or 8) and x-chunk XC (a power of two dividing N, at least
-// kGaugeG): workgroups and partial-sum rows; false when N does not fit the tile.
+// Shape of the tiled pass with TY-row tiles (4 or 8) and x-chunk XC (a power of two dividing N, a multiple
+// of the row group G): workgroups and partial-sum rows; false when N does not fit the tile.
 bool gauge_tiled_shape(int N, int ty, int XC, int *blocks, int *rows)
 {
   if ((ty != 4 && ty != 8) || N % GTZ != 0 || N % ty != 0) return false;
-  if (XC < kGaugeG || XC > N || N % XC != 0 || (XC & (XC - 1)) != 0) return false;
+  const int G = gauge_rows_g(N);
+  if (XC < G || XC > N || N % XC != 0 || XC % G != 0 || (XC & (XC - 1)) != 0) return false;
   const long tiles = (long)(N / ty) * (N / GTZ);
   *blocks = (int)(tiles * (N / XC));
-  *rows = (int)(tiles * (N / kGaugeG));
+  *rows = (int)(tiles * (N / G));
   return true;
 }
 
@@ -291,7 +296,7 @@ bool gauge_tiled_shape(int N, int ty, int XC, int *blocks, int *rows)
 bool gauge_tiled_plan(int N, int ty, int ncu, int *XC, int *blocks, int *rows)
 {
   int best = 0;
-  for (int xc = kGaugeG; xc <= N; xc *= 2) {
+  for (int xc = 8; xc <= N; xc *= 2) {
     int b = 0, r = 0;
     if (!gauge_tiled_shape(N, ty, xc, &b, &r)) continue;
     if (best == 0 || b >= 2 * ncu) best = xc;

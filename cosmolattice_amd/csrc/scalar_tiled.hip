@@ -11,6 +11,8 @@
 // Workgroups are dispatched round-robin over the 8 XCDs; the block index is remapped so that the
 // workgroups of one XCD own neighbouring tiles and x-chunks, which keeps halo re-reads in that
 // XCD's L2.
+#include <algorithm>
+
 #include "clmi_internal.hpp"
 #include "scalar_poly.hpp"
 
@@ -337,7 +339,8 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 {
   constexpr int TY = 16;
   if (nf < 1 || nf > 2 || N % TZ != 0 || N % TY != 0) return false;
-  const int G = N <= 256 ? 4 : N / 64;
+  // planes per partial-sum row: fewer rows for the background update to sum (1024 at 256^3)
+  const int G = N <= 256 ? std::max(4, std::min(16, N / 16)) : N / 64;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
   const long long tiles = (long long)(N / TZ) * (N / TY);
   p->XC = XC;

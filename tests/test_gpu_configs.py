@@ -128,7 +128,7 @@ def test_c3_u1_256_autotune_keeps_state_and_results(gpu, prec):
     """At 256^3 the autotuner times every x-chunk of the tiled charged-scalar + U(1) pass on the live
     lattice without changing it, and the chunk it picks gives fields, momenta and the background (volume
     sums included) bit-identical to the default chunk over 6 leapfrog steps with expansion: the pass
-    writes one partial row per group of 8 planes whatever the chunk."""
+    writes one partial row per group of 32 planes (at 256^3) whatever the chunk."""
     say(f"case c3 autotune prec={prec}")
     outs = []
     for tune in (False, True):
@@ -187,7 +187,11 @@ def test_c3_u1_256_long_run_constraints(gpu):
     """lphi4U1 at 256^3 over 2000 leapfrog steps with expansion, FP64 and FP32-bulk from the same
     initial conditions: the Gauss violation stays at rounding level (FP64 < 1e-12, FP32-bulk < 1e-5), the
     Friedmann (energy-constraint) violation of both modes stays below 1e-3 and FP32-bulk's differs from
-    FP64's by less than 1e-5 at every check (every 250 steps), and the scale factors agree to 1e-8."""
+    FP64's by less than 1e-5 at every check (every 250 steps), and the scale factors agree to 2^-24.
+    FP32-bulk forms each site's energy terms in FP32 (summed over sites in FP64); the lattice is close to
+    homogeneous, so their rounding is nearly the same at every site and does not average out: the energy
+    density, and through the Friedmann equation the scale factor (a ~ 190 here), carries a relative error
+    of order 2^-24."""
     say("case c3 long run")
     recs = {}
     for prec in (CLMI_F64, CLMI_F32_BULK):
@@ -207,7 +211,7 @@ def test_c3_u1_256_long_run_constraints(gpu):
     assert np.all(np.abs(r64[:, 0]) < 1e-3) and np.all(np.abs(r32[:, 0]) < 1e-3)
     assert np.max(np.abs(r32[:, 0] - r64[:, 0])) < 1e-5
     assert r64[-1, 2] > 1.0
-    assert np.max(np.abs(r32[:, 2] - r64[:, 2]) / r64[:, 2]) < 1e-8
+    assert np.max(np.abs(r32[:, 2] - r64[:, 2]) / r64[:, 2]) < 2.0**-24
 
 
 # ---- batched ensemble ------------------------------------------------------------------------------
