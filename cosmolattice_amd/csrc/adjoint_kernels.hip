@@ -257,11 +257,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_finish(const double *partials, i
 }
 
 // The host arithmetic of the reverse sweep's background chain, moved to the device so that the sweep
-// needs no host round trip (see launch_adj_chain_fields / launch_adj_chain_kick).
-__global__ void k_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
-                                   double w, double V, const ModelConsts mc, int nterms, int ex)
+// needs no host round trip (launch_adj_chain_fields, launch_adj_finish_chains).
+__device__ void chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n, double w,
+                             double V, const ModelConsts &mc, int nterms, int ex)
 {
-  if (threadIdx.x != 0) return;
   const double dt = mc.dt, al = mc.alpha;
   const clmi_background &bn = tape[n], &bn1 = tape[n + 1];
   for (int t = 0; t < nterms; ++t) S->gc[t] += S->lam_U * red[fo + 1 + t] / V;
@@ -296,10 +295,9 @@ __global__ void k_adj_chain_fields(AdjScalars *S, const double *red, int fo, con
   S->aV = pow(bn.aI, 3 + al);
 }
 
-__global__ void k_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
-                                 const ModelConsts mc, int nterms, int ex)
+__device__ void chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n, const ModelConsts &mc,
+                           int nterms, int ex)
 {
-  if (threadIdx.x != 0) return;
   const double al = mc.alpha, aI = tape[n].aI, ck = S->ck, aV = S->aV;
   for (int t = 0; t < nterms; ++t) S->gc[t] += -ck * aV * red[2 + t];
   if (ex) {
@@ -311,6 +309,41 @@ __global__ void k_adj_chain_kick(AdjScalars *S, const double *red, const clmi_ba
   }
 }
 
+__global__ void k_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
+                                   double w, double V, const ModelConsts mc, int nterms, int ex)
+{
+  if (threadIdx.x == 0) chain_fields(S, red, fo, tape, n, w, V, mc, nterms, ex);
+}
+
+// One reverse step's scalar work in one launch: the fixed-order sum of the fused kick + fields pass's
+// partial rows (as k_adj_finish, also written to out), then chain_kick of step n and, for n > 0,
+// chain_fields of step n - 1 on the fields sums.
+__global__ __launch_bounds__(kBlock) void k_adj_finish_chains(const double *partials, int rows, double *out,
+                                                              AdjScalars *S, const clmi_background *tape, int n,
+                                                              double w, double V, const ModelConsts mc, int nterms,
+                                                              int ex)
+{
+  constexpr int NG = kBlock / kAdjRed;
+  __shared__ double sh[NG][kAdjRed];
+  __shared__ double red[kAdjRed];
+  const int k = threadIdx.x % kAdjRed, g = threadIdx.x / kAdjRed;
+  double s = 0.0;
+#pragma unroll 8
+  for (int r = g; r < rows; r += NG) s += partials[(size_t)r * kAdjRed + k];
+  sh[g][k] = s;
+  __syncthreads();
+  if (threadIdx.x < kAdjRed) {
+    double t = 0.0;
+    for (int q = 0; q < NG; ++q) t += sh[q][threadIdx.x];
+    red[threadIdx.x] = t;
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  chain_kick(S, red, tape, n, mc, nterms, ex);
+  if (n > 0) chain_fields(S, red, kAdjFieldSlots, tape, n - 1, w, V, mc, nterms, ex);
+}
+
 }  // namespace
 
 hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
@@ -320,17 +353,18 @@ hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, con
   return hipGetLastError();
 }
 
-hipError_t launch_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
-                                 const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_adj_chain_kick, dim3(1), dim3(64), 0, s, S, red, tape, n, mc, nterms, (int)ex);
-  return hipGetLastError();
-}
-
 int adj_blocks(uint32_t V)
 {
   const uint32_t b = (V + kBlock - 1) / kBlock;
   return (int)(b < 1024 ? b : 1024);
+}
+
+hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_background *tape, int n, double w,
+                                    double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adj_finish_chains, dim3(1), dim3(kBlock), 0, s, (const double *)a.partials, adj_blocks(a.V),
+                     a.red_out, S, tape, n, w, V, mc, nterms, (int)ex);
+  return hipGetLastError();
 }
 
 namespace {
@@ -396,9 +430,8 @@ hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, doub
 hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, hipStream_t s)
 {
   if (!a.sc) return hipErrorInvalidValue;
-  hipError_t e = with_fields ? adj_dispatch<KickFieldsK>(a, f32, s, 0.0, 0.0, 0.0)
-                             : adj_dispatch<KickK>(a, f32, s, 0.0, 0.0, 0.0);
-  return e == hipSuccess ? adj_finish(a, s) : e;
+  return with_fields ? adj_dispatch<KickFieldsK>(a, f32, s, 0.0, 0.0, 0.0)
+                     : adj_dispatch<KickK>(a, f32, s, 0.0, 0.0, 0.0);
 }
 
 }  // namespace clmi

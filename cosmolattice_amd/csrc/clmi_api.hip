@@ -1208,19 +1208,26 @@ struct AdjState {
   DevArr f[CLMI_MAX_SCALARS], p[CLMI_MAX_SCALARS];
 };
 
-int adj_alloc(clmi_ctx *c, AdjState &s)
+// The adjoint's states live in one device allocation, carved into arrays of 256-byte aligned size.
+size_t adj_arr_bytes(size_t b) { return (b + 255) / 256 * 256; }
+size_t adj_state_bytes(const clmi_ctx *c)
 {
-  for (int f = 0; f < c->cfg.n_scalars; ++f) {
-    CHK(alloc_arr(c, s.f[f], true));
-    CHK(alloc_arr(c, s.p[f], false));
-  }
-  return CLMI_OK;
+  const size_t bf = adj_arr_bytes(bytes_field(c->f32) * c->V);
+  const size_t bl = c->f32 ? adj_arr_bytes(sizeof(float) * c->V) : 0;
+  return (size_t)c->cfg.n_scalars * (2 * bf + bl);
 }
-void adj_free(AdjState &s)
+void adj_carve(const clmi_ctx *c, char *&p, AdjState &s)
 {
-  for (int f = 0; f < CLMI_MAX_SCALARS; ++f) {
-    free_arr(s.f[f]);
-    free_arr(s.p[f]);
+  const size_t bf = adj_arr_bytes(bytes_field(c->f32) * c->V);
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    s.f[f].p = p;
+    p += bf;
+    if (c->f32) {
+      s.f[f].lo = (float *)p;
+      p += adj_arr_bytes(sizeof(float) * c->V);
+    }
+    s.p[f].p = p;
+    p += bf;
   }
 }
 // One copy launch for a whole state (every array of every scalar singlet).
@@ -1263,10 +1270,46 @@ AdjFields adj_view(const clmi_ctx *c, const AdjState &s)
   return v;
 }
 
+// The context's scalar-singlet buffers, restored when the recomputation of a segment ends (it points
+// the context's spare buffers at segment slots).
+struct AdjBufferGuard {
+  clmi_ctx *c;
+  DevArr f[CLMI_MAX_SCALARS][2], p[CLMI_MAX_SCALARS][2];
+  explicit AdjBufferGuard(clmi_ctx *ctx) : c(ctx)
+  {
+    for (int i = 0; i < CLMI_MAX_SCALARS; ++i)
+      for (int b = 0; b < 2; ++b) {
+        f[i][b] = c->fS[i][b];
+        p[i][b] = c->pS[i][b];
+      }
+  }
+  ~AdjBufferGuard()
+  {
+    for (int i = 0; i < CLMI_MAX_SCALARS; ++i)
+      for (int b = 0; b < 2; ++b) {
+        c->fS[i][b] = f[i][b];
+        c->pS[i][b] = p[i][b];
+      }
+  }
+};
+
+AdjFields adj_view_cur(const clmi_ctx *c)
+{
+  AdjFields v{};
+  for (int f = 0; f < c->cfg.n_scalars; ++f) {
+    v.phi[f] = c->fS[f][c->cur_fS].p;
+    v.philo[f] = c->fS[f][c->cur_fS].lo;
+    v.pi[f] = c->pS[f][c->cur_pS].p;
+  }
+  return v;
+}
+
 struct AdjRun {
   clmi_ctx *c = nullptr;
   std::vector<AdjState> checkpoints, segment;
+  std::vector<AdjFields> seg_view;  // state after step n0 + i of the segment being reversed
   AdjState final_state;
+  void *pool = nullptr;        // every state's arrays
   double *lam = nullptr;       // 2 * n_scalars FP64 arrays
   double *partials = nullptr;  // 1024 x kAdjRed
   double *red = nullptr;       // kAdjRed
@@ -1274,9 +1317,7 @@ struct AdjRun {
   clmi_background *tape = nullptr;  // device copy of the forward background after every step
   ~AdjRun()
   {
-    for (auto &s : checkpoints) adj_free(s);
-    for (auto &s : segment) adj_free(s);
-    adj_free(final_state);
+    if (pool) (void)hipFree(pool);
     if (lam) (void)hipFree(lam);
     if (partials) (void)hipFree(partials);
     if (red) (void)hipFree(red);
@@ -1320,10 +1361,16 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   run.c = c;
   const int ncp = (nsteps + M - 1) / M;
   run.checkpoints.resize(ncp);
-  run.segment.resize(M + 1);
-  for (auto &s : run.checkpoints) CHK(adj_alloc(c, s));
-  for (auto &s : run.segment) CHK(adj_alloc(c, s));
-  CHK(adj_alloc(c, run.final_state));
+  run.segment.resize(M);
+  run.seg_view.resize(M);
+  {
+    const size_t sb = adj_state_bytes(c);
+    HIPCHK(c, hipMalloc(&run.pool, sb * (size_t)(ncp + M + 1)));
+    char *p = (char *)run.pool;
+    for (auto &s : run.checkpoints) adj_carve(c, p, s);
+    for (auto &s : run.segment) adj_carve(c, p, s);
+    adj_carve(c, p, run.final_state);
+  }
   HIPCHK(c, hipMalloc(&run.lam, sizeof(double) * 2 * nf * c->V));
   HIPCHK(c, hipMalloc(&run.partials, sizeof(double) * 1024 * kAdjRed));
   HIPCHK(c, hipMalloc(&run.red, sizeof(double) * kAdjRed));
@@ -1453,31 +1500,42 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
     CHK(adj_restore(c, run.checkpoints[k]));
     CHK(clmi_set_background(c, &tape[n0]));
     c->synced = (n0 == 0);
-    for (int n = n0; n < n1; ++n) {
-      CHK(clmi_lf_evolve(c, 1));
-      CHK(adj_save(c, run.segment[n - n0]));
+    {
+      // Each drift + kick step writes its state straight into segment slot n - n0: the context's spare
+      // buffers point there for the step.  The first step of a segment is a kick alone, which updates
+      // the momenta in place, so that state stays in the context's own buffers, which no later step of
+      // the segment writes.
+      AdjBufferGuard guard(c);
+      for (int n = n0; n < n1; ++n) {
+        if (c->pending_drift) {
+          const AdjState &q = run.segment[n - n0];
+          for (int f = 0; f < nf; ++f) {
+            c->fS[f][1 - c->cur_fS] = q.f[f];
+            c->pS[f][1 - c->cur_pS] = q.p[f];
+          }
+        }
+        CHK(clmi_lf_evolve(c, 1));
+        run.seg_view[n - n0] = adj_view_cur(c);
+      }
     }
     if (k == ncp - 1) {
       // storeFieldsAverages of the final phi (unchanged by the sync), then the drift's sum
       // lam_phi . pi_{T-1/2}
       ad.fwd = adj_view(c, run.final_state);
-      ad.fwd_red = adj_view(c, run.segment[n1 - 1 - n0]);
+      ad.fwd_red = run.seg_view[n1 - 1 - n0];
       HIPCHK(c, launch_adj_fields(ad, c->f32, 0.0, 0.0, 0.0, false, c->stream));
       HIPCHK(c, launch_adj_chain_fields(run.sc, run.red, 0, run.tape, nsteps - 1, nsteps == 1 ? 0.5 : 1.0, V, mc, nt,
                                         ex, c->stream));
     }
     for (int n = n1 - 1; n >= n0; --n) {
       // drift and <pi^2> transposes of step n (pi_{n+1/2})
-      ad.fwd = adj_view(c, run.segment[n - n0]);
+      ad.fwd = run.seg_view[n - n0];
       ad.fwd_red = AdjFields{};
       HIPCHK(c, launch_adj_pi(ad, c->f32, 0.0, 0.0, c->stream));
       // kick of step n on phi_n, with the fields transpose of phi_n for step n - 1
-      ad.fwd_red = adj_view(c, n > n0 ? run.segment[n - 1 - n0] : run.checkpoints[k]);
+      ad.fwd_red = n > n0 ? run.seg_view[n - 1 - n0] : adj_view(c, run.checkpoints[k]);
       HIPCHK(c, launch_adj_kick_fields(ad, c->f32, n > 0, c->stream));
-      HIPCHK(c, launch_adj_chain_kick(run.sc, run.red, run.tape, n, mc, nt, ex, c->stream));
-      if (n > 0)
-        HIPCHK(c, launch_adj_chain_fields(run.sc, run.red, kAdjFieldSlots, run.tape, n - 1, n - 1 == 0 ? 0.5 : 1.0, V,
-                                          mc, nt, ex, c->stream));
+      HIPCHK(c, launch_adj_finish_chains(ad, run.sc, run.tape, n, n - 1 == 0 ? 0.5 : 1.0, V, mc, nt, ex, c->stream));
     }
   }
   c->pending_drift = false;

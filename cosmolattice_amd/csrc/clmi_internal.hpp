@@ -255,25 +255,27 @@ __host__ __device__ inline SFd sf_partials(const ModelConsts &mc, double a, doub
   d.da = (2 * al + 1) / a * A * B + A * ((al - 2) * 0.5 * pi2 * (-6.0) * pow(a, -7.0) + al * 0.5 * G * (-2.0) * pow(a, -3.0));
   return d;
 }
-// Scalar steps of the reverse sweep, one thread each, on the stream after the lattice kernel whose
-// sums (red) they consume.  chain_fields: after the fields transpose of state n+1 -- derivatives by
-// the potential coefficients, the background adjoint of the drift and scale-factor update of step n,
-// and the coefficients of the pi and kick transposes of step n (w: the kick weight, V: N^3; the
-// fields sums sit at red[fo], red[fo + 1 + t]).  chain_kick:
-// after the kick transpose of step n -- its a_I and coefficient terms, then the step's background
-// adjoints become the current ones.
+// Scalar steps of the reverse sweep, one thread, on the stream after the lattice kernel whose sums
+// (red) they consume.  chain_fields: after the fields transpose of state n+1 -- derivatives by the
+// potential coefficients, the background adjoint of the drift and scale-factor update of step n, and
+// the coefficients of the pi and kick transposes of step n (w: the kick weight, V: N^3; the fields
+// sums sit at red[fo], red[fo + 1 + t]).  chain_kick: after the kick transpose of step n -- its a_I
+// and coefficient terms, then the step's background adjoints become the current ones.
 hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, const clmi_background *tape, int n,
                                    double w, double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
-hipError_t launch_adj_chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n,
-                                 const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
+// After launch_adj_kick_fields of step n: the fixed-order sum of its partial rows into a.red_out, then
+// chain_kick of step n and, for n > 0, chain_fields of step n - 1 (weight w) on the fields sums, on the
+// background adjoint S (the a.sc the lattice kernels read), in one launch.
+hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_background *tape, int n, double w,
+                                    double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
 hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lamU, double lamP, bool with_pi,
                              hipStream_t s);
 hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s);
 hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s);
 // Kick transpose of step n fused with the fields transpose of the same state (coefficients from
-// a.sc): the sums of the kick at slots 0 .. 1 + n_terms, those of the fields transpose at
-// kAdjFieldSlots (lam_phi . pi) and kAdjFieldSlots + 1 + t (monomials).  with_fields = false: the
-// kick transpose alone.
+// a.sc): per-block partial sums of the kick at slots 0 .. 1 + n_terms, those of the fields transpose
+// at kAdjFieldSlots (lam_phi . pi) and kAdjFieldSlots + 1 + t (monomials); launch_adj_finish_chains
+// sums them.  with_fields = false: the kick transpose alone.
 hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, hipStream_t s);
 
 // Launchers (defined in the .hip files).
