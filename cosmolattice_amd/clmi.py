@@ -142,6 +142,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "clmi_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "clmi_copy_calibration": (C.c_int, [P, C.c_size_t, C.c_int]),
         "clmi_autotune": (C.c_int, [P, C.c_int, C.c_char_p, C.c_size_t]),
+        "clmi_plan_report": (C.c_int, [P, C.c_char_p, C.c_size_t]),
         "clmi_field_stats_noalloc": (C.c_int, [P, C.c_int, C.c_double, P, P]),
         "clmi_noalloc_last_error": (C.c_char_p, []),
     }
@@ -513,6 +514,12 @@ class Lattice:
     def autotune(self, reps: int = 5) -> str:
         buf = C.create_string_buffer(4096)
         self._check(self.lib.clmi_autotune(self._h, reps, buf, len(buf)), "autotune")
+        return buf.value.decode()
+
+    def plan(self) -> str:
+        """The lattice passes this context runs, one line per sector (clmi_plan_report)."""
+        buf = C.create_string_buffer(1024)
+        self._check(self.lib.clmi_plan_report(self._h, buf, len(buf)), "plan_report")
         return buf.value.decode()
 
     def copy_calibration(self, nbytes: int, iters: int = 3):

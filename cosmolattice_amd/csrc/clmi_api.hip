@@ -2087,6 +2087,35 @@ int clmi_autotune(clmi_ctx *c, int reps, char *report, size_t report_len)
   return CLMI_OK;
 }
 
+int clmi_plan_report(const clmi_ctx *c, char *report, size_t report_len)
+{
+  if (!c) return CLMI_E_INVALID;
+  if (!report || report_len == 0) return CLMI_E_INVALID;
+  std::string rep;
+  if (c->cfg.n_scalars > 0) {
+    if (c->rowk)
+      rep += "k_scalar_rows VZ=" + std::to_string(c->rplan.VZ) + " WPR=" + std::to_string(c->rplan.WPR) +
+             " TY=" + std::to_string(c->rplan.TY) + " XC=" + std::to_string(c->rplan.XC) +
+             " MM=" + std::to_string(c->rplan.MM) +
+             " PK=" + std::to_string(scalar_rows_pk(c->cfg.n_scalars, c->f32, c->spoly, c->rplan)) + "\n";
+    else if (c->tiled)
+      rep += "k_scalar_tiled XC=" + std::to_string(c->tplan.XC) + " G=" + std::to_string(c->tplan.G) +
+             " NT=" + std::to_string((int)c->tplan.NT) + " PK=" + std::to_string(poly_kind(c->spoly, c->cfg.n_scalars)) +
+             "\n";
+    else
+      rep += "k_scalar_pass\n";
+  }
+  if (c->cfg.n_cscalars > 0) {
+    if (c->gtiled)
+      rep += "k_gauge_tiled TY=" + std::to_string(c->gTY) + " XC=" + std::to_string(c->gXC) + "\n";
+    else
+      rep += "k_gauge_pass\n";
+  }
+  std::strncpy(report, rep.c_str(), report_len - 1);
+  report[report_len - 1] = '\0';
+  return rep.size() < report_len ? CLMI_OK : CLMI_E_INVALID;
+}
+
 int clmi_copy_calibration(clmi_ctx *c, size_t bytes, int iters)
 {
   CHK(check_ctx(c));

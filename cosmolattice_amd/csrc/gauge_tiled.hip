@@ -19,8 +19,15 @@ namespace clmi {
 namespace {
 
 constexpr int GTZ = 64;  // z sites per tile (one per lane)
-// planes per partial-sum row: N/8 between 8 and 32 (2048 rows at 256^3 for the background update)
-inline int gauge_rows_g(int N) { return std::max(8, std::min(32, N / 8)); }
+// planes per partial-sum row: the largest power of two (so that it divides N, a multiple of 64) not
+// above N/8, between 8 and 32 (2048 rows at 256^3 for the background update)
+inline int gauge_rows_g(int N)
+{
+  const int target = std::min(32, N / 8);
+  int G = 8;
+  while (2 * G <= target) G *= 2;
+  return G;
+}
 constexpr int GNC = 5;   // components: phi_0, phi_1, A_x, A_y, A_z
 
 // Raw (not yet drifted) values of one site's five components.

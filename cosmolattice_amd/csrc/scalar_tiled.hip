@@ -339,8 +339,12 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 {
   constexpr int TY = 16;
   if (nf < 1 || nf > 2 || N % TZ != 0 || N % TY != 0) return false;
-  // planes per partial-sum row: fewer rows for the background update to sum (1024 at 256^3)
-  const int G = N <= 256 ? std::max(4, std::min(16, N / 16)) : N / 64;
+  // planes per partial-sum row: a power of two (so that it divides N, a multiple of 64), the largest
+  // not above N/16 (at most 16) up to 256^3 and not above N/64 beyond, at least 4: few rows for the
+  // background update to sum (1024 at 256^3), and at most 64 row groups per tile column
+  const int target = N <= 256 ? std::min(16, N / 16) : N / 64;
+  int G = 4;
+  while (2 * G <= target) G *= 2;
   if (XC < G || XC % G != 0 || N % XC != 0) return false;
   const long long tiles = (long long)(N / TZ) * (N / TY);
   p->XC = XC;

@@ -305,6 +305,12 @@ int clmi_adjoint_lf(clmi_ctx *ctx, int nsteps, int checkpoint_every, int bg_from
  * the volume sums.  A no-op returning CLMI_OK where no tiled pass applies. */
 int clmi_autotune(clmi_ctx *ctx, int reps, char *report, size_t report_len);
 
+/* No counterpart in the reference.  Names the lattice passes the context runs, one line per sector
+ * ("k_scalar_rows VZ=.. WPR=.. TY=.. XC=.. MM=.. PK=..", "k_scalar_tiled XC=.. G=.. NT=.. PK=..",
+ * "k_scalar_pass"; "k_gauge_tiled TY=.. XC=..", "k_gauge_pass"), into report (NUL-terminated).
+ * Launches nothing.  CLMI_E_INVALID if report is NULL or too short for the whole text. */
+int clmi_plan_report(const clmi_ctx *ctx, char *report, size_t report_len);
+
 /* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */
 int clmi_profile_enable(clmi_ctx *ctx, int on);
 /* Total milliseconds and number of timed launches since enabling; bytes_per_launch is the

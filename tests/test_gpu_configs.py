@@ -114,6 +114,9 @@ def test_c3_u1_256_tiled_bit_identical_to_generic(gpu, prec):
     res = []
     for flags in (0, CLMI_FLAG_NO_TILED):
         _, lat, _ = u1_vacuum(prec, expansion=False, flags=flags)
+        plan = lat.plan()
+        say("plan", plan.replace("\n", "; "))
+        assert ("k_gauge_tiled" if flags == 0 else "k_gauge_pass") in plan
         lat.lf_init()
         lat.lf_evolve(7)
         lat.lf_sync()

@@ -63,6 +63,29 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+def expected_scalar_pass(N, prec, flags=0):
+    """The scalar lattice pass a context of this size, precision and flags runs: the full-row pass for
+    64, 128, 192, 256 and 512 (FP32-bulk) except FP64 at 256^3 without a memory-mode or shape flag, the
+    tiled pass for the other multiples of 64, and the generic pass with CLMI_FLAG_NO_TILED."""
+    from cosmolattice_amd.clmi import CLMI_FLAG_NO_TILED
+
+    if flags & CLMI_FLAG_NO_TILED:
+        return "k_scalar_pass"
+    asked = (flags >> 8) & 15 or flags & 6
+    if N in (64, 128, 192) or (N == 256 and (prec == CLMI_F32_BULK or asked)) or (N == 512 and prec == CLMI_F32_BULK):
+        return "k_scalar_rows"
+    return "k_scalar_tiled" if N % 64 == 0 else "k_scalar_pass"
+
+
+def assert_plan(lat, kernel):
+    """The context runs `kernel` (first word of a line of clmi_plan_report), so a comparison with the
+    generic pass is not the generic pass against itself."""
+    plan = lat.plan()
+    say("plan", plan.replace("\n", "; "))
+    assert kernel in [ln.split()[0] for ln in plan.splitlines()], plan
+    return plan
+
+
 def run_reference_loop(obj, steps, out_every, evolver="LF", ws=(1.0,)):
     recs = []
     if evolver == "LF":
@@ -138,6 +161,7 @@ def test_fast_pass_bit_identical_to_generic(gpu, N, prec, flags):
     res = []
     for flags in (flags, CLMI_FLAG_NO_TILED):
         lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
+        assert_plan(lat, expected_scalar_pass(N, prec, flags))
         for (k, i), a in ics.items():
             lat.upload(k, i, 0, a)
         lat.set_background(aI=1.1, aSI=1.05)
@@ -177,6 +201,8 @@ def test_fast_pass_runtime_potential_bit_identical_to_generic(gpu, N, prec):
     res = []
     for flags in (0, CLMI_FLAG_NO_TILED):
         lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
+        plan = assert_plan(lat, expected_scalar_pass(N, prec, flags))
+        assert flags or "PK=0" in plan
         for (k, i), a in ics.items():
             lat.upload(k, i, 0, a)
         lat.set_background(aI=1.1, aSI=1.05)
@@ -229,6 +255,7 @@ def test_verlet_passes_bit_identical_to_generic(gpu, N, prec, mm):
         res = []
         for flags in (CLMI_FLAG_MEMMODE(mm) if mm else 0, CLMI_FLAG_NO_TILED):
             lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=False, flags=flags)
+            assert_plan(lat, expected_scalar_pass(N, prec, flags))
             for (k, i), a in ics.items():
                 lat.upload(k, i, 0, a)
             lat.set_background(aI=1.1, aSI=1.05, aIM=1.0)
@@ -598,7 +625,8 @@ def test_pv_lphi4u1_fp64_parity(gpu, oracle_mod, N):
 
 @pytest.mark.parametrize("N,prec,ty8", [(64, CLMI_F64, False), (64, CLMI_F32_BULK, False),
                                         (128, CLMI_F32_BULK, False), (128, CLMI_F64, False),
-                                        (64, CLMI_F64, True), (128, CLMI_F32_BULK, True)])
+                                        (64, CLMI_F64, True), (128, CLMI_F32_BULK, True),
+                                        (192, CLMI_F64, False), (192, CLMI_F32_BULK, False)])
 def test_tiled_gauge_pass_bit_identical_to_generic(gpu, N, prec, ty8):
     """The tiled charged-scalar + U(1) pass gives the same fields, bit for bit, as the
     one-site-per-thread pass (flag CLMI_FLAG_NO_TILED); volume averages agree to summation order."""
@@ -612,6 +640,8 @@ def test_tiled_gauge_pass_bit_identical_to_generic(gpu, N, prec, ty8):
     res = []
     for flags in (CLMI_FLAG_GAUGE_TY8 if ty8 else 0, CLMI_FLAG_NO_TILED):
         lat = Lattice(spec, N, dx, dt, prec, expansion=False, flags=flags)
+        plan = assert_plan(lat, "k_gauge_pass" if flags == CLMI_FLAG_NO_TILED else "k_gauge_tiled")
+        assert flags == CLMI_FLAG_NO_TILED or f"TY={8 if ty8 else 4}" in plan
         load_u1(lat, st)
         lat.set_background(aI=1.0, aSI=1.0, aIM=1.0, aDotI=0.0)
         lat.lf_init()
@@ -641,6 +671,7 @@ def test_tiled_gauge_verlet_bit_identical_to_generic(gpu, N, prec):
         res = []
         for flags in (0, CLMI_FLAG_NO_TILED):
             lat = Lattice(spec, N, 0.6, 0.01, prec, expansion=False, flags=flags)
+            assert_plan(lat, "k_gauge_tiled" if flags == 0 else "k_gauge_pass")
             load_u1(lat, st)
             lat.set_background(aI=1.1, aSI=1.05, aIM=1.0, aDotI=0.0)
             if scheme == "vv":
