@@ -731,16 +731,25 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
   const bool ex = c->cfg.expansion != 0;
   const double dt = c->cfg.dt;
   const int stages = (nws - 1) * 2 + 1;
+  auto weight = [&](int s) { return ws[s < nws ? s : stages - s - 1]; };
   for (int n = 0; n < nsteps; ++n)
     for (int s = 0; s < stages; ++s) {
-      const double w = ws[s < nws ? s : stages - s - 1];
-      if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
+      const double w = weight(s);
+      // the scale-factor half kick that opens a stage; after the first it rides on the previous stage's
+      // last update (no lattice work lies between them, so the scalar sequence is the same)
+      if (ex && n == 0 && s == 0) CHK(update(c, 0, {{OP_VV_SF_HALF, w}}));
       CHK(pass_update(c, false, true, ex, 0.0, w * dt / 2, dt / 2 * w, ex,
                       {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, w}}));
       // drift and second half-kick in one pass: the drift factor (a_SI) and the kick factors (a_I) are both
       // set by the update above, and the pass's sums are the drifted fields' and the kicked momenta's
-      CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
-                      {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
+      const bool last = n == nsteps - 1 && s == stages - 1;
+      if (ex && !last)
+        CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
+                        {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w},
+                         {OP_VV_SF_HALF, weight((s + 1) % stages)}}));
+      else
+        CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
+                        {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
     }
   return CLMI_OK;
 }
@@ -753,15 +762,22 @@ int clmi_pv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
   const bool ex = c->cfg.expansion != 0;
   const double dt = c->cfg.dt;
   const int stages = (nws - 1) * 2 + 1;
+  auto weight = [&](int s) { return ws[s < nws ? s : stages - s - 1]; };
   if (!ex) CHK(update(c, 0, {{OP_PV_DRIFT_AI, 0.0}}));
   for (int n = 0; n < nsteps; ++n)
     for (int s = 0; s < stages; ++s) {
-      const double w = ws[s < nws ? s : stages - s - 1];
-      // drift (aI of the stage start) + kick (half-step aI) in one pass, then the mid-stage update
-      if (ex) CHK(update(c, 0, {{OP_PV_SF_FIRST, w}}));
+      const double w = weight(s);
+      // drift (aI of the stage start) + kick (half-step aI) in one pass, then the mid-stage update; the
+      // stage's first scale-factor drift rides on the previous stage's last update after the first stage
+      if (ex && n == 0 && s == 0) CHK(update(c, 0, {{OP_PV_SF_FIRST, w}}));
       CHK(pass_update(c, true, true, ex, dt / 2 * w, w * dt, w * dt, ex, {{OP_PV_MID, w}}));
       // second drift (new aI) + storeFieldsAveragesInt
-      CHK(pass_update(c, true, false, ex, dt / 2 * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
+      const bool last = n == nsteps - 1 && s == stages - 1;
+      if (ex && !last)
+        CHK(pass_update(c, true, false, ex, dt / 2 * w, 0.0, 0.0, ex,
+                        {{OP_STORE_FIELDS, 0.0}, {OP_PV_SF_FIRST, weight((s + 1) % stages)}}));
+      else
+        CHK(pass_update(c, true, false, ex, dt / 2 * w, 0.0, 0.0, ex, {{OP_STORE_FIELDS, 0.0}}));
     }
   // static background: restore the aSI-based drift coefficients of the other evolvers (with
   // expansion the last update already did)
