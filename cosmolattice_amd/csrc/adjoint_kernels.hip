@@ -235,6 +235,215 @@ __global__ __launch_bounds__(kBlock) void k_adj_kick(const AdjArgs a, double c, 
   block_reduce_store<kAdjRed>(acc, a.partials + (size_t)blockIdx.x * kAdjRed);
 }
 
+// Tiled form of (C), for N a multiple of 64, one or two fields and the compile-time lphi4 potential
+// (with a runtime exponent table every term's accumulator stays live and the kernel runs out of
+// registers, so those runs keep k_adj_kick): a workgroup owns a tile of kAdjTY
+// y-rows x 64 z-sites (each wavefront kAdjRY rows, one z-site per lane) and marches along x over XC
+// planes.  lam_pi and phi of the x-neighbours come from a register queue (planes x-1, x, x+1; plane
+// x+2 is loaded while x is computed), those of the y- and z-neighbours from a double-buffered LDS
+// tile of the plane with a one-site halo.  Every value is read from memory once per tile instead of
+// seven times; the per-site arithmetic, term order included, is k_adj_kick's, so lam_phi is
+// bit-identical to it.  The pointwise inputs of a plane (lam_phi, the momenta of the fields sum) are
+// loaded one plane ahead.  One partial row per workgroup.
+constexpr int kAdjRY = 1, kAdjTY = 4 * kAdjRY, kAdjTZ = 64;
+
+template <int NF, bool F32, bool FIELDS, int PK>
+__global__ __launch_bounds__(kBlock) void k_adj_kick_tiled(const AdjArgs a, double c, double aK, double aV,
+                                                           const int XC)
+{
+  static_assert(kBlock == 4 * 64, "four wavefronts per tile");
+  __shared__ double lamL[2][NF][kAdjTY + 2][kAdjTZ + 2];
+  __shared__ double phiL[2][NF][kAdjTY + 2][kAdjTZ + 2];
+  double lamG = 0.0, lamU = 0.0;
+  if (a.sc) {
+    c = a.sc->ck;
+    aK = a.sc->aK;
+    aV = a.sc->aV;
+    lamG = a.sc->fG;
+    lamU = a.sc->fU;
+  }
+  const int nt = n_terms<NF, PK>(a.poly);
+  const double invV = 1.0 / (double)a.V;
+  const int N = (int)a.N;
+  const uint32_t NN = (uint32_t)N * N;
+  const int nTz = N / kAdjTZ, nTy = N / kAdjTY;
+  const int b = blockIdx.x;
+  const int tz = b % nTz, ty = (b / nTz) % nTy, cx = b / (nTz * nTy);
+  const int z0 = tz * kAdjTZ, y0 = ty * kAdjTY, x0 = cx * XC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int z = z0 + lane;
+  auto site = [&](int x, int y, int zz) -> uint32_t {
+    x = x < 0 ? x + N : (x >= N ? x - N : x);
+    y = y < 0 ? y + N : (y >= N ? y - N : y);
+    zz = zz < 0 ? zz + N : (zz >= N ? zz - N : zz);
+    return (uint32_t)x * NN + (uint32_t)y * N + zz;
+  };
+  const AdjFields &fr = a.fwd_red.pi[0] ? a.fwd_red : a.fwd;
+
+  // halo lanes: wave 0 -> row y0-1, wave 1 -> row y0+TY, waves 2/3 lanes < TY -> columns z0-1 / z0+TZ
+  int hy = 0, hz = 0, hly = 0, hlz = 0;
+  bool has_halo = false;
+  if (wave == 0) { hy = y0 - 1; hz = z; hly = 0; hlz = lane + 1; has_halo = true; }
+  else if (wave == 1) { hy = y0 + kAdjTY; hz = z; hly = kAdjTY + 1; hlz = lane + 1; has_halo = true; }
+  else if (lane < kAdjTY) {
+    hy = y0 + lane;
+    hly = lane + 1;
+    if (wave == 2) { hz = z0 - 1; hlz = 0; } else { hz = z0 + kAdjTZ; hlz = kAdjTZ + 1; }
+    has_halo = true;
+  }
+
+  double lprev[kAdjRY][NF], lcur[kAdjRY][NF], lnxt[kAdjRY][NF];
+  double pprev[kAdjRY][NF], pcur[kAdjRY][NF], pnxt[kAdjRY][NF];
+  double lphi[kAdjRY][NF], pr[kAdjRY][NF];  // pointwise inputs of the plane being computed
+  double acc[kAdjRed];
+#pragma unroll
+  for (int k = 0; k < kAdjRed; ++k) acc[k] = 0.0;
+
+  // prologue: planes x0-1, x0, x0+1 of the own rows, the pointwise inputs of x0, the tile of x0
+#pragma unroll
+  for (int r = 0; r < kAdjRY; ++r) {
+    const int y = y0 + wave * kAdjRY + r;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const uint32_t sm = site(x0 - 1, y, z), s0 = site(x0, y, z), sp = site(x0 + 1, y, z);
+      lprev[r][f] = a.lam_pi[f][sm];
+      pprev[r][f] = ld_phi<F32>(a.fwd, f, sm);
+      lcur[r][f] = a.lam_pi[f][s0];
+      pcur[r][f] = ld_phi<F32>(a.fwd, f, s0);
+      lnxt[r][f] = a.lam_pi[f][sp];
+      pnxt[r][f] = ld_phi<F32>(a.fwd, f, sp);
+      lphi[r][f] = a.lam_phi[f][s0];
+      pr[r][f] = FIELDS ? ld_pi<F32>(fr, f, s0) : 0.0;
+      lamL[0][f][wave * kAdjRY + r + 1][lane + 1] = lcur[r][f];
+      phiL[0][f][wave * kAdjRY + r + 1][lane + 1] = pcur[r][f];
+    }
+  }
+  if (has_halo) {
+    const uint32_t sh = site(x0, hy, hz);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      lamL[0][f][hly][hlz] = a.lam_pi[f][sh];
+      phiL[0][f][hly][hlz] = ld_phi<F32>(a.fwd, f, sh);
+    }
+  }
+  __syncthreads();
+
+  for (int i = 0; i < XC; ++i) {
+    const int x = x0 + i;
+    const int buf = i & 1;
+    const bool more = i + 1 < XC;
+    // loads for later planes: own rows at x+2, pointwise inputs at x+1, halo at x+1
+    double lf[kAdjRY][NF], pf[kAdjRY][NF], lphin[kAdjRY][NF], prn[kAdjRY][NF], hl[NF], hp[NF];
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < kAdjRY; ++r) {
+        const int y = y0 + wave * kAdjRY + r;
+        const uint32_t s2 = site(x + 2, y, z), s1 = site(x + 1, y, z);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          lf[r][f] = a.lam_pi[f][s2];
+          pf[r][f] = ld_phi<F32>(a.fwd, f, s2);
+          lphin[r][f] = a.lam_phi[f][s1];
+          prn[r][f] = FIELDS ? ld_pi<F32>(fr, f, s1) : 0.0;
+        }
+      }
+      if (has_halo) {
+        const uint32_t sh = site(x + 1, hy, hz);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          hl[f] = a.lam_pi[f][sh];
+          hp[f] = ld_phi<F32>(a.fwd, f, sh);
+        }
+      }
+    }
+
+#pragma unroll
+    for (int r = 0; r < kAdjRY; ++r) {
+      const int ly = wave * kAdjRY + r + 1;
+      const uint32_t s = site(x, y0 + ly - 1, z);
+      double phi[NF], lp[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        phi[f] = pcur[r][f];
+        lp[f] = lcur[r][f];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double lapl = (lnxt[r][f] + lprev[r][f] - 2 * lp[f] + lamL[buf][f][ly + 1][lane + 1] +
+                             lamL[buf][f][ly - 1][lane + 1] - 2 * lp[f] + lamL[buf][f][ly][lane + 2] +
+                             lamL[buf][f][ly][lane] - 2 * lp[f]) *
+                            a.inv_dx2;
+        double hsum = 0.0;
+#pragma unroll
+        for (int g = 0; g < NF; ++g) {
+          double h = 0.0;
+#pragma unroll
+          for (int t = 0; t < max_terms<NF, PK>(); ++t)
+            if (t < nt) h += a.poly.c[t] * d2mono<NF, PK>(a.poly, t, phi, f, g);
+          hsum += h * lp[g];
+        }
+        double l = lphi[r][f] + c * (aK * lapl - aV * hsum);
+        const double c0 = phi[f];
+        const double lapf = (pnxt[r][f] + pprev[r][f] - 2 * c0 + phiL[buf][f][ly + 1][lane + 1] +
+                             phiL[buf][f][ly - 1][lane + 1] - 2 * c0 + phiL[buf][f][ly][lane + 2] +
+                             phiL[buf][f][ly][lane] - 2 * c0) *
+                            a.inv_dx2;
+        double dv = 0.0;
+#pragma unroll
+        for (int t = 0; t < max_terms<NF, PK>(); ++t) {
+          if (t >= nt) break;
+          const double dm = dmono<NF, PK>(a.poly, t, phi, f);
+          dv += a.poly.c[t] * dm;
+          if (t + 2 < kAdjFieldSlots) acc[2 + t] += lp[f] * dm;
+        }
+        acc[0] += lp[f] * lapf;
+        acc[1] += lp[f] * dv;
+        if (FIELDS) {
+          l += lamG * (-2.0 * invV) * lapf + lamU * dv * invV;
+          acc[kAdjFieldSlots] += l * pr[r][f];
+        }
+        a.lam_phi[f][s] = l;
+      }
+      if (FIELDS)
+#pragma unroll
+        for (int t = 0; t < max_terms<NF, PK>(); ++t)
+          if (t < nt && kAdjFieldSlots + 1 + t < kAdjRed) acc[kAdjFieldSlots + 1 + t] += mono<NF, PK>(a.poly, t, phi);
+    }
+
+    if (more) {
+      // tile of plane x+1, then advance the register queues
+#pragma unroll
+      for (int r = 0; r < kAdjRY; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          lamL[buf ^ 1][f][wave * kAdjRY + r + 1][lane + 1] = lnxt[r][f];
+          phiL[buf ^ 1][f][wave * kAdjRY + r + 1][lane + 1] = pnxt[r][f];
+        }
+      if (has_halo)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          lamL[buf ^ 1][f][hly][hlz] = hl[f];
+          phiL[buf ^ 1][f][hly][hlz] = hp[f];
+        }
+#pragma unroll
+      for (int r = 0; r < kAdjRY; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          lprev[r][f] = lcur[r][f];
+          lcur[r][f] = lnxt[r][f];
+          lnxt[r][f] = lf[r][f];
+          pprev[r][f] = pcur[r][f];
+          pcur[r][f] = pnxt[r][f];
+          pnxt[r][f] = pf[r][f];
+          lphi[r][f] = lphin[r][f];
+          pr[r][f] = prn[r][f];
+        }
+    }
+    __syncthreads();
+  }
+  block_reduce_store<kAdjRed>(acc, a.partials + (size_t)b * kAdjRed);
+}
+
 // Fixed-order sum of `rows` partial rows into out[0..kAdjRed): thread (g, k) adds slot k of rows g,
 // g + NG, ... (coalesced across a wavefront, eight loads in flight), then the NG group sums are added
 // in group order.
@@ -353,19 +562,31 @@ hipError_t launch_adj_chain_fields(AdjScalars *S, const double *red, int fo, con
   return hipGetLastError();
 }
 
+// Plan of the tiled kick transpose: x-chunk XC (the shortest from 4 up that leaves at most 512
+// workgroups) and workgroups, one partial row each; false where it does not apply (N not a multiple of
+// 64, more than two fields, a runtime potential table, more rows than the partial buffer holds, or the
+// caller asks for the one-site-per-thread kernel).
+bool adj_kick_tiled_plan(const AdjArgs &a, int *XC, int *blocks)
+{
+  const int N = (int)a.N;
+  if (a.no_tiled || a.nf < 1 || a.nf > 2 || N % kAdjTZ != 0 || N % kAdjTY != 0) return false;
+  if (poly_kind(a.poly, a.nf) != 1) return false;
+  const long tiles = (long)(N / kAdjTZ) * (N / kAdjTY);
+  int xc = 4;
+  while (tiles * (N / xc) > 512 && 2 * xc <= N && N % (2 * xc) == 0) xc *= 2;
+  const long b = tiles * (N / xc);
+  if (b > kAdjMaxRows) return false;
+  *XC = xc;
+  *blocks = (int)b;
+  return true;
+}
+
 int adj_blocks(uint32_t V)
 {
   const uint32_t b = (V + kBlock - 1) / kBlock;
-  return (int)(b < 1024 ? b : 1024);
+  return (int)(b < (uint32_t)kAdjMaxRows ? b : (uint32_t)kAdjMaxRows);
 }
 
-hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_background *tape, int n, double w,
-                                    double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_adj_finish_chains, dim3(1), dim3(kBlock), 0, s, (const double *)a.partials, adj_blocks(a.V),
-                     a.red_out, S, tape, n, w, V, mc, nterms, (int)ex);
-  return hipGetLastError();
-}
 
 namespace {
 
@@ -401,10 +622,34 @@ template <int NF, bool F32, int PK> struct KickFieldsK {
   static constexpr auto kernel = k_adj_kick<NF, F32, true, PK>;
 };
 
-hipError_t adj_finish(const AdjArgs &a, hipStream_t s)
+hipError_t adj_finish(const AdjArgs &a, int rows, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, adj_blocks(a.V), a.red_out);
+  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, rows, a.red_out);
   return hipGetLastError();
+}
+
+// Partial rows the kick transpose writes (the tiled or the one-site-per-thread kernel).
+int adj_kick_rows(const AdjArgs &a)
+{
+  int xc = 0, b = 0;
+  return adj_kick_tiled_plan(a, &xc, &b) ? b : adj_blocks(a.V);
+}
+
+// The kick transpose with coefficients (c, aK, aV) or, with a.sc set, from a.sc.
+template <bool FIELDS>
+hipError_t adj_kick_dispatch(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s)
+{
+  int xc = 0, b = 0;
+  if (!adj_kick_tiled_plan(a, &xc, &b))
+    return FIELDS ? adj_dispatch<KickFieldsK>(a, f32, s, c, aK, aV) : adj_dispatch<KickK>(a, f32, s, c, aK, aV);
+#define CLMI_ADJT(NF, F)                                                                                      \
+  if (a.nf == NF && f32 == F) {                                                                                \
+    hipLaunchKernelGGL((k_adj_kick_tiled<NF, F, FIELDS, 1>), dim3(b), dim3(kBlock), 0, s, a, c, aK, aV, xc);  \
+    return hipGetLastError();                                                                                  \
+  }
+  CLMI_ADJT(1, false) CLMI_ADJT(1, true) CLMI_ADJT(2, false) CLMI_ADJT(2, true)
+#undef CLMI_ADJT
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -413,7 +658,7 @@ hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lam
                              hipStream_t s)
 {
   hipError_t e = adj_dispatch<FieldsK>(a, f32, s, lamG, lamU, lamP, (int)with_pi);
-  return e == hipSuccess ? adj_finish(a, s) : e;
+  return e == hipSuccess ? adj_finish(a, adj_blocks(a.V), s) : e;
 }
 
 hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s)
@@ -423,15 +668,22 @@ hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hip
 
 hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s)
 {
-  hipError_t e = adj_dispatch<KickK>(a, f32, s, c, aK, aV);
-  return e == hipSuccess ? adj_finish(a, s) : e;
+  hipError_t e = adj_kick_dispatch<false>(a, f32, c, aK, aV, s);
+  return e == hipSuccess ? adj_finish(a, adj_kick_rows(a), s) : e;
 }
 
 hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, hipStream_t s)
 {
   if (!a.sc) return hipErrorInvalidValue;
-  return with_fields ? adj_dispatch<KickFieldsK>(a, f32, s, 0.0, 0.0, 0.0)
-                     : adj_dispatch<KickK>(a, f32, s, 0.0, 0.0, 0.0);
+  return with_fields ? adj_kick_dispatch<true>(a, f32, 0.0, 0.0, 0.0, s) : adj_kick_dispatch<false>(a, f32, 0.0, 0.0, 0.0, s);
+}
+
+hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_background *tape, int n, double w,
+                                    double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adj_finish_chains, dim3(1), dim3(kBlock), 0, s, (const double *)a.partials, adj_kick_rows(a),
+                     a.red_out, S, tape, n, w, V, mc, nterms, (int)ex);
+  return hipGetLastError();
 }
 
 }  // namespace clmi

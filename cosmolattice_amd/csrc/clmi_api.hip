@@ -1327,7 +1327,7 @@ struct AdjRun {
   AdjState final_state;
   void *pool = nullptr;        // every state's arrays
   double *lam = nullptr;       // 2 * n_scalars FP64 arrays
-  double *partials = nullptr;  // 1024 x kAdjRed
+  double *partials = nullptr;  // kAdjMaxRows x kAdjRed
   double *red = nullptr;       // kAdjRed
   AdjScalars *sc = nullptr;    // background adjoint during the reverse sweep
   clmi_background *tape = nullptr;  // device copy of the forward background after every step
@@ -1388,7 +1388,7 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
     adj_carve(c, p, run.final_state);
   }
   HIPCHK(c, hipMalloc(&run.lam, sizeof(double) * 2 * nf * c->V));
-  HIPCHK(c, hipMalloc(&run.partials, sizeof(double) * 1024 * kAdjRed));
+  HIPCHK(c, hipMalloc(&run.partials, sizeof(double) * kAdjMaxRows * kAdjRed));
   HIPCHK(c, hipMalloc(&run.red, sizeof(double) * kAdjRed));
   HIPCHK(c, hipMalloc(&run.sc, sizeof(AdjScalars)));
   HIPCHK(c, hipMalloc(&run.tape, sizeof(clmi_background) * (size_t)(nsteps + 1)));
@@ -1448,6 +1448,7 @@ int clmi_adjoint_lf(clmi_ctx *c, int nsteps, int checkpoint_every, int bg_from_f
   a.poly = c->spoly;
   a.partials = run.partials;
   a.red_out = run.red;
+  a.no_tiled = (c->cfg.flags & CLMI_FLAG_NO_TILED) ? 1 : 0;
   for (int f = 0; f < nf; ++f) {
     a.lam_phi[f] = run.lam + (size_t)f * c->V;
     a.lam_pi[f] = run.lam + (size_t)(nf + f) * c->V;
@@ -2126,6 +2127,19 @@ int clmi_plan_report(const clmi_ctx *c, char *report, size_t report_len)
       rep += "k_gauge_tiled TY=" + std::to_string(c->gTY) + " XC=" + std::to_string(c->gXC) + "\n";
     else
       rep += "k_gauge_pass\n";
+  } else if (c->cfg.n_scalars > 0) {
+    // the kick transpose of clmi_adjoint_lf
+    AdjArgs a{};
+    a.nf = c->cfg.n_scalars;
+    a.N = (uint32_t)c->N;
+    a.V = c->V;
+    a.poly = c->spoly;
+    a.no_tiled = (c->cfg.flags & CLMI_FLAG_NO_TILED) ? 1 : 0;
+    int xc = 0, blocks = 0;
+    if (adj_kick_tiled_plan(a, &xc, &blocks))
+      rep += "k_adj_kick_tiled XC=" + std::to_string(xc) + " blocks=" + std::to_string(blocks) + "\n";
+    else
+      rep += "k_adj_kick\n";
   }
   std::strncpy(report, rep.c_str(), report_len - 1);
   report[report_len - 1] = '\0';

@@ -205,6 +205,7 @@ hipError_t launch_update_batch(const UpdMember *d_members, int n, const UpdateOp
 
 // ---- adjoint (adjoint_kernels.hip) ----
 constexpr int kAdjRed = 32;  // partial-sum slots of the adjoint reductions
+constexpr int kAdjMaxRows = 1024;  // partial rows (one per workgroup) of an adjoint lattice kernel
 constexpr int kAdjFieldSlots = 16;  // first slot of the fields-transpose sums in the fused reverse pass
 struct AdjFields {           // forward fields of one leapfrog state, in storage precision
   const void *phi[CLMI_MAX_SCALARS];
@@ -232,9 +233,10 @@ struct AdjArgs {
   uint32_t N, V;
   double inv_dx2;
   ScalarPoly poly;
-  double *partials;  // >= 1024 rows of kAdjRed
+  double *partials;  // kAdjMaxRows rows of kAdjRed
   double *red_out;   // kAdjRed finished sums
   const AdjScalars *sc;  // non-null: the kernels take their coefficients from here, not from arguments
+  int no_tiled;      // 1: the kick transpose runs one site per thread even where the tiled form applies
 };
 
 // ScaleFactorKernels::get for scalar singlets (evolvers/kernels/scalefactorkernels.h:27-81) and its
@@ -270,6 +272,9 @@ hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_
                                     double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s);
 hipError_t launch_adj_fields(const AdjArgs &a, bool f32, double lamG, double lamU, double lamP, bool with_pi,
                              hipStream_t s);
+// Plan of the tiled kick transpose for a.N, a.nf, a.poly and a.no_tiled: x-chunk and workgroups;
+// false where the one-site-per-thread kernel runs instead.
+bool adj_kick_tiled_plan(const AdjArgs &a, int *XC, int *blocks);
 hipError_t launch_adj_pi(const AdjArgs &a, bool f32, double cF, double lamP, hipStream_t s);
 hipError_t launch_adj_kick(const AdjArgs &a, bool f32, double c, double aK, double aV, hipStream_t s);
 // Kick transpose of step n fused with the fields transpose of the same state (coefficients from

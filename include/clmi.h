@@ -307,8 +307,10 @@ int clmi_autotune(clmi_ctx *ctx, int reps, char *report, size_t report_len);
 
 /* No counterpart in the reference.  Names the lattice passes the context runs, one line per sector
  * ("k_scalar_rows VZ=.. WPR=.. TY=.. XC=.. MM=.. PK=..", "k_scalar_tiled XC=.. G=.. NT=.. PK=..",
- * "k_scalar_pass"; "k_gauge_tiled TY=.. XC=..", "k_gauge_pass"), into report (NUL-terminated).
- * Launches nothing.  CLMI_E_INVALID if report is NULL or too short for the whole text. */
+ * "k_scalar_pass"; "k_gauge_tiled TY=.. XC=..", "k_gauge_pass"; for scalar-singlet models the kick
+ * transpose of clmi_adjoint_lf, "k_adj_kick_tiled XC=.. blocks=.." or "k_adj_kick"), into report
+ * (NUL-terminated).  Launches nothing.  CLMI_E_INVALID if report is NULL or too short for the whole
+ * text. */
 int clmi_plan_report(const clmi_ctx *ctx, char *report, size_t report_len);
 
 /* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */
