@@ -444,25 +444,43 @@ __global__ __launch_bounds__(kBlock) void k_adj_kick_tiled(const AdjArgs a, doub
   block_reduce_store<kAdjRed>(acc, a.partials + (size_t)b * kAdjRed);
 }
 
-// Fixed-order sum of `rows` partial rows into out[0..kAdjRed): thread (g, k) adds slot k of rows g,
-// g + NG, ... (coalesced across a wavefront, eight loads in flight), then the NG group sums are added
-// in group order.
-__global__ __launch_bounds__(kBlock) void k_adj_finish(const double *partials, int rows, double *out)
+// Fixed-order sum of `rows` partial rows into red[0..kAdjRed) (shared memory) by kFinThreads threads:
+// thread (g, k) adds slot k of rows g, g + NG, ... in that order (coalesced across a wavefront, the
+// loads of kFinU rows in flight at once), then the NG group sums are added in group order.
+constexpr int kFinThreads = 1024, kFinU = 16;
+__device__ __forceinline__ void finish_rows(const double *partials, int rows, double *red)
 {
-  static_assert(kBlock % kAdjRed == 0, "row groups");
-  constexpr int NG = kBlock / kAdjRed;
+  static_assert(kFinThreads % kAdjRed == 0, "row groups");
+  constexpr int NG = kFinThreads / kAdjRed;
   __shared__ double sh[NG][kAdjRed];
   const int k = threadIdx.x % kAdjRed, g = threadIdx.x / kAdjRed;
   double s = 0.0;
-#pragma unroll 8
-  for (int r = g; r < rows; r += NG) s += partials[(size_t)r * kAdjRed + k];
+  for (int r0 = g; r0 < rows; r0 += kFinU * NG) {
+    double q[kFinU];
+#pragma unroll
+    for (int u = 0; u < kFinU; ++u) {
+      const int r = r0 + u * NG;
+      q[u] = r < rows ? partials[(size_t)r * kAdjRed + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFinU; ++u)
+      if (r0 + u * NG < rows) s += q[u];
+  }
   sh[g][k] = s;
   __syncthreads();
   if (threadIdx.x < kAdjRed) {
     double t = 0.0;
     for (int q = 0; q < NG; ++q) t += sh[q][threadIdx.x];
-    out[threadIdx.x] = t;
+    red[threadIdx.x] = t;
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_adj_finish(const double *partials, int rows, double *out)
+{
+  __shared__ double red[kAdjRed];
+  finish_rows(partials, rows, red);
+  if (threadIdx.x < kAdjRed) out[threadIdx.x] = red[threadIdx.x];
 }
 
 // The host arithmetic of the reverse sweep's background chain, moved to the device so that the sweep
@@ -473,13 +491,13 @@ __device__ void chain_fields(AdjScalars *S, const double *red, int fo, const clm
   const double dt = mc.dt, al = mc.alpha;
   const clmi_background &bn = tape[n], &bn1 = tape[n + 1];
   for (int t = 0; t < nterms; ++t) S->gc[t] += S->lam_U * red[fo + 1 + t] / V;
-  const double F = pow(bn1.aSI, al - 3);
+  const double F = rpow(bn1.aSI, al - 3);
   double lamP = 0.0;
   S->fG = S->lam_G;
   S->fU = S->lam_U;
   if (ex) {
     // drift: aSI' = (aI + aI')/2 enters F; aI' = aI + dt aDotSI'; aDotSI' = aDotSI + dt K
-    const double lam_aSI1 = dt * red[fo] * (al - 3) * pow(bn1.aSI, al - 4);
+    const double lam_aSI1 = dt * red[fo] * (al - 3) * rpow(bn1.aSI, al - 4);
     const double lam_aI1 = S->lam_aI + 0.5 * lam_aSI1;
     double n_aI = 0.5 * lam_aSI1 + lam_aI1;
     const double lam_aDotSI1 = S->lam_aDotSI + dt * lam_aI1;
@@ -500,8 +518,8 @@ __device__ void chain_fields(AdjScalars *S, const double *red, int fo, const clm
   S->cF = dt * F;
   S->lamP = lamP;
   S->ck = w * dt;
-  S->aK = pow(bn.aI, 1 + al);
-  S->aV = pow(bn.aI, 3 + al);
+  S->aK = rpow(bn.aI, 1 + al);
+  S->aV = rpow(bn.aI, 3 + al);
 }
 
 __device__ void chain_kick(AdjScalars *S, const double *red, const clmi_background *tape, int n, const ModelConsts &mc,
@@ -510,7 +528,7 @@ __device__ void chain_kick(AdjScalars *S, const double *red, const clmi_backgrou
   const double al = mc.alpha, aI = tape[n].aI, ck = S->ck, aV = S->aV;
   for (int t = 0; t < nterms; ++t) S->gc[t] += -ck * aV * red[2 + t];
   if (ex) {
-    S->lam_aI = S->n_aI + ck * (red[0] * (1 + al) * pow(aI, al) - red[1] * (3 + al) * pow(aI, 2 + al));
+    S->lam_aI = S->n_aI + ck * (red[0] * (1 + al) * rpow(aI, al) - red[1] * (3 + al) * rpow(aI, 2 + al));
     S->lam_aDotSI = S->n_aDotSI;
     S->lam_pi2SI = S->n_pi2SI;
     S->lam_G = S->n_G;
@@ -527,27 +545,14 @@ __global__ void k_adj_chain_fields(AdjScalars *S, const double *red, int fo, con
 // One reverse step's scalar work in one launch: the fixed-order sum of the fused kick + fields pass's
 // partial rows (as k_adj_finish, also written to out), then chain_kick of step n and, for n > 0,
 // chain_fields of step n - 1 on the fields sums.
-__global__ __launch_bounds__(kBlock) void k_adj_finish_chains(const double *partials, int rows, double *out,
-                                                              AdjScalars *S, const clmi_background *tape, int n,
-                                                              double w, double V, const ModelConsts mc, int nterms,
-                                                              int ex)
+__global__ __launch_bounds__(kFinThreads) void k_adj_finish_chains(const double *partials, int rows, double *out,
+                                                                   AdjScalars *S, const clmi_background *tape, int n,
+                                                                   double w, double V, const ModelConsts mc,
+                                                                   int nterms, int ex)
 {
-  constexpr int NG = kBlock / kAdjRed;
-  __shared__ double sh[NG][kAdjRed];
   __shared__ double red[kAdjRed];
-  const int k = threadIdx.x % kAdjRed, g = threadIdx.x / kAdjRed;
-  double s = 0.0;
-#pragma unroll 8
-  for (int r = g; r < rows; r += NG) s += partials[(size_t)r * kAdjRed + k];
-  sh[g][k] = s;
-  __syncthreads();
-  if (threadIdx.x < kAdjRed) {
-    double t = 0.0;
-    for (int q = 0; q < NG; ++q) t += sh[q][threadIdx.x];
-    red[threadIdx.x] = t;
-    out[threadIdx.x] = t;
-  }
-  __syncthreads();
+  finish_rows(partials, rows, red);
+  if (threadIdx.x < kAdjRed) out[threadIdx.x] = red[threadIdx.x];
   if (threadIdx.x != 0) return;
   chain_kick(S, red, tape, n, mc, nterms, ex);
   if (n > 0) chain_fields(S, red, kAdjFieldSlots, tape, n - 1, w, V, mc, nterms, ex);
@@ -624,7 +629,7 @@ template <int NF, bool F32, int PK> struct KickFieldsK {
 
 hipError_t adj_finish(const AdjArgs &a, int rows, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kBlock), 0, s, a.partials, rows, a.red_out);
+  hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kFinThreads), 0, s, a.partials, rows, a.red_out);
   return hipGetLastError();
 }
 
@@ -681,7 +686,7 @@ hipError_t launch_adj_kick_fields(const AdjArgs &a, bool f32, bool with_fields, 
 hipError_t launch_adj_finish_chains(const AdjArgs &a, AdjScalars *S, const clmi_background *tape, int n, double w,
                                     double V, const ModelConsts &mc, int nterms, bool ex, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_adj_finish_chains, dim3(1), dim3(kBlock), 0, s, (const double *)a.partials, adj_kick_rows(a),
+  hipLaunchKernelGGL(k_adj_finish_chains, dim3(1), dim3(kFinThreads), 0, s, (const double *)a.partials, adj_kick_rows(a),
                      a.red_out, S, tape, n, w, V, mc, nterms, (int)ex);
   return hipGetLastError();
 }

@@ -9,24 +9,6 @@
 
 namespace clmi {
 
-// a^e: integer exponents (|e| <= 32, the common case alpha = 1) by binary powering, which is a
-// few multiplications instead of a serial log/exp evaluation on the update's single thread;
-// anything else by pow().  Agrees with pow() to a few ulp.
-__device__ inline double rpow(double a, double e)
-{
-  if (e == rint(e) && fabs(e) <= 32.0) {
-    int n = (int)fabs(e);
-    double p = 1.0, q = a;
-    while (n) {
-      if (n & 1) p *= q;
-      q *= q;
-      n >>= 1;
-    }
-    return e < 0 ? 1.0 / p : p;
-  }
-  return pow(a, e);
-}
-
 // The scale-factor kernel of ScaleFactorKernels::get/getSI (evolvers/kernels/scalefactorkernels.h:27-124)
 // with the energy normalisations of definitions/energies.h:37-85, from the given volume averages.
 __device__ inline double sf_kernel_of(double a, double pi2S, double grad2S, double pi2CS, double grad2CS,
