@@ -61,10 +61,15 @@ def make_lattice(clmi, spec, N, kIR, dt, precision, seed, flags=0, round_pi_f32=
     return lat, dx
 
 
+# HIP events time one drift + kick pass in PROF_EVERY: timing every pass adds two event records per step,
+# about 7 us at 256^3, to the throughput being measured.
+PROF_EVERY = 16
+
+
 def timed_run(lat, warmup, steps):
     lat.lf_evolve(warmup)
     lat.synchronize()
-    lat.profile_enable(True)
+    lat.profile_enable(True, every=PROF_EVERY)
     t0 = time.perf_counter()
     lat.lf_evolve(steps)
     lat.synchronize()
@@ -604,8 +609,8 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "traffic_source": TRAFFIC_FILE if traffic is not None else None,
                 "kernel": r["kernel"] + " (fused drift+kick+reductions)",
-                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": r["bytes_per_launch"],
-                "autotune_us": r["autotune_us"]}
+                "avg_launch_us": avg_s * 1e6, "timed_launches": r["launches"], "timed_every": PROF_EVERY,
+                "bytes_per_launch": r["bytes_per_launch"], "autotune_us": r["autotune_us"]}
 
     if args.calibrate_copy:
         # A plain copy kernel of known size (k_copy): its FETCH_SIZE / WRITE_SIZE counters

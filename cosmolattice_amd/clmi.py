@@ -501,8 +501,9 @@ class Lattice:
                 gb.as_dict())
 
     # -- timing ------------------------------------------------------------------------------------
-    def profile_enable(self, on: bool = True):
-        self._check(self.lib.clmi_profile_enable(self._h, 1 if on else 0), "profile_enable")
+    def profile_enable(self, on: bool = True, every: int = 1):
+        """HIP-event timing of every `every`-th drift + kick pass (clmi_profile_enable); off with on=False."""
+        self._check(self.lib.clmi_profile_enable(self._h, max(1, int(every)) if on else 0), "profile_enable")
 
     def profile_read(self) -> tuple:
         ms = C.c_double()

@@ -77,6 +77,8 @@ struct clmi_ctx {
 
   // profiling
   bool prof = false;
+  int prof_every = 1;     // time every prof_every-th drift + kick pass
+  int64_t prof_count = 0; // drift + kick passes since enabling
   std::vector<hipEvent_t> ev;  // pairs
   size_t ev_used = 0;
   double prof_ms = 0.0;
@@ -252,7 +254,7 @@ GaugePassArgs gauge_args(const clmi_ctx *c, int fi, int fo, int pi, int po, doub
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
 int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
 {
-  const bool timed = c->prof && drift && kick;
+  const bool timed = c->prof && drift && kick && (c->prof_count++ % c->prof_every == 0);
   if (timed) CHK(prof_event(c));
   int r = 0;
   if (c->cfg.n_scalars > 0) {
@@ -1958,7 +1960,9 @@ int clmi_profile_enable(clmi_ctx *c, int on)
 {
   CHK(check_ctx(c));
   CHK(prof_flush(c));
-  c->prof = on != 0;
+  c->prof = on > 0;
+  c->prof_every = on > 0 ? on : 1;
+  c->prof_count = 0;
   c->prof_ms = 0.0;
   c->prof_launches = 0;
   return CLMI_OK;

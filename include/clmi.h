@@ -313,7 +313,9 @@ int clmi_autotune(clmi_ctx *ctx, int reps, char *report, size_t report_len);
  * text. */
 int clmi_plan_report(const clmi_ctx *ctx, char *report, size_t report_len);
 
-/* Enable HIP-event timing of the dominant per-step lattice pass (on the context stream). */
+/* HIP-event timing of the dominant per-step lattice pass (the drift + kick pass, on the context
+ * stream): on >= 1 times every on-th such pass (1: every pass; a larger period keeps the events'
+ * own cost out of a throughput measurement), on <= 0 switches timing off.  Resets the totals. */
 int clmi_profile_enable(clmi_ctx *ctx, int on);
 /* Total milliseconds and number of timed launches since enabling; bytes_per_launch is the
  * algorithmic HBM traffic of one such pass (every evolved array read once and written once). */
