@@ -948,3 +948,33 @@ def test_ensemble_rejects_unsupported_members(gpu):
             Ensemble(group)
     for lat in (a, b, c):
         lat.close()
+
+
+def test_profile_timing_samples_one_pass_in_n(gpu):
+    """clmi_profile_enable(ctx, n) times every n-th drift + kick pass with HIP events (n = 1: every pass):
+    12 fused leapfrog steps give 12 timed launches with n = 1 and 3 with n = 4, and the average duration
+    is positive; the fields do not depend on the timing."""
+    say("case profile sampling")
+    spec = bench_spec()
+    N = 64
+    ics = scalar_ics(spec, N, np.random.default_rng(12))
+    outs = []
+    for every in (1, 4):
+        lat = Lattice(spec, N, 0.4909, 0.005, CLMI_F64, expansion=False)
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.set_background(aI=1.1, aSI=1.05)
+        lat.lf_init()
+        lat.lf_evolve(1)
+        lat.profile_enable(True, every=every)
+        lat.lf_evolve(12)
+        ms, launches, bpl = lat.profile_read()
+        lat.profile_enable(False)
+        say("every", every, "launches", launches, "ms", ms)
+        assert launches == (12 if every == 1 else 3)
+        assert ms > 0 and bpl > 0
+        lat.lf_sync()
+        outs.append([lat.download(k, i, 0) for (k, i) in sorted(ics)])
+        lat.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
