@@ -37,14 +37,20 @@ __global__ __launch_bounds__(kUpdThreads) void k_update_batch(const UpdMember *m
   const UpdMember &u = m[blockIdx.x];
   __shared__ double tot[kNumRed];
   const int t = threadIdx.x;
+  // the updating thread copies the member's state and constants into registers at entry, so that their
+  // loads overlap the partial-row loads instead of sitting in the scalar chain
   DevState s;
-  if (t == 0) s = *u.st;
+  ModelConsts mc;
+  if (t == 0) {
+    s = *u.st;
+    mc = u.mc;
+  }
   if (u.rows > 0) sum_partials<kUpdThreads>(u.partials, u.rows, tot);
   if (t != 0) return;
   double red[kNumRed];
-  for (int k = 0; k < kNumRed; ++k) red[k] = u.rows > 0 ? tot[k] * u.mc.invV : s.red[k];
+  for (int k = 0; k < kNumRed; ++k) red[k] = u.rows > 0 ? tot[k] * mc.invV : s.red[k];
   for (int k = 0; k < kNumRed; ++k) s.red[k] = red[k];
-  apply_update(&s, red, u.mc, ops);
+  apply_update(&s, red, mc, ops);
   *u.st = s;
 }
 

@@ -41,8 +41,8 @@ __device__ inline double sf_kernel(const clmi_background &b, const ModelConsts &
 }
 
 // Fixed-order sum of `rows` partial rows by a block of NT threads: thread t adds rows t, t + NT, ... in
-// that order, eight rows in flight per iteration (each row one 64-byte line, read as four 16-byte
-// loads), then wavefront shuffles, then the wavefront sums in order.  Result in out[0..kNumRed)
+// that order, four rows in flight per batch (each row one 64-byte line, read as four 16-byte loads),
+// then wavefront shuffles, then the wavefront sums in order.  Result in out[0..kNumRed)
 // (shared memory); ends with a barrier.  The rows are 64-byte aligned (kNumRed = 8 doubles).
 template <int NT>
 __device__ inline void sum_partials(const double *partials, int rows, double *out)
@@ -55,29 +55,25 @@ __device__ inline void sum_partials(const double *partials, int rows, double *ou
   double v[kNumRed];
 #pragma unroll
   for (int k = 0; k < kNumRed; ++k) v[k] = 0.0;
-  int r = t;
-  constexpr int U = 8;
-  for (; r + (U - 1) * NT < rows; r += U * NT) {
+  // rows t, t + NT, t + 2 NT, ... in that order, U rows per batch with all their loads in flight (a
+  // thread has at most U rows below U * NT rows, so those take one memory round trip)
+  constexpr int U = 4;
+  for (int r0 = t; r0 < rows; r0 += U * NT) {
     d2 q[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u)
+      if (r0 + u * NT < rows)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) q[u][k] = P[(size_t)(r + u * NT) * 4 + k];
+        for (int k = 0; k < 4; ++k) q[u][k] = P[(size_t)(r0 + u * NT) * 4 + k];
 #pragma unroll
     for (int u = 0; u < U; ++u)
+      if (r0 + u * NT < rows)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        v[2 * k] += q[u][k].x;
-        v[2 * k + 1] += q[u][k].y;
-      }
+        for (int k = 0; k < 4; ++k) {
+          v[2 * k] += q[u][k].x;
+          v[2 * k + 1] += q[u][k].y;
+        }
   }
-  for (; r < rows; r += NT)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const d2 q = P[(size_t)r * 4 + k];
-      v[2 * k] += q.x;
-      v[2 * k + 1] += q.y;
-    }
 #pragma unroll
   for (int k = 0; k < kNumRed; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
