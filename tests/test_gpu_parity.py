@@ -978,3 +978,32 @@ def test_profile_timing_samples_one_pass_in_n(gpu):
         lat.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+def test_invalid_arguments_are_rejected_with_a_message(gpu):
+    """The C ABI refuses what it cannot run, with a message and without touching the device state:
+    lattice sizes outside even 4..1024, a negative step count, the adjoint of a model with a complex scalar,
+    unknown memory modes, and a plan-report buffer too short for the text."""
+    import ctypes as C
+
+    from cosmolattice_amd.clmi import CLMI_FLAG_MEMMODE, ClmiError
+
+    say("case invalid arguments")
+    spec = bench_spec()
+    for n in (2, 33, 1026):
+        with pytest.raises(ClmiError, match="N must be even"):
+            Lattice(spec, n, 0.5, 0.005, CLMI_F64)
+    with pytest.raises(ClmiError, match="memory mode"):
+        Lattice(spec, 48, 0.5, 0.005, CLMI_F32_BULK, flags=CLMI_FLAG_MEMMODE(15))
+    lat = Lattice(spec, 16, 0.5, 0.005, CLMI_F64)
+    with pytest.raises(ClmiError, match="nsteps"):
+        lat.lf_evolve(-1)
+    buf = C.create_string_buffer(8)
+    assert lat.lib.clmi_plan_report(lat._h, buf, len(buf)) != 0
+    assert lat.plan().startswith("k_scalar_pass")
+    lat.close()
+    u1 = Lattice(u1_spec(), 16, 0.6, 0.01, CLMI_F64)
+    z = [np.zeros((16,) * 3)] * u1.spec.n_scalars
+    with pytest.raises((ClmiError, ValueError)):
+        u1.adjoint_lf(3, z, z, (0.0, 0.0))
+    u1.close()
