@@ -61,15 +61,20 @@ def make_lattice(clmi, spec, N, kIR, dt, precision, seed, flags=0, round_pi_f32=
     return lat, dx
 
 
-# HIP events time one drift + kick pass in PROF_EVERY: timing every pass adds two event records per step,
-# about 7 us at 256^3, to the throughput being measured.
+# HIP events time one drift + kick pass in prof_every(steps): timing every pass adds two event records per
+# step, about 7 us at 256^3, to the throughput being measured.  Short runs time every pass, so that the
+# kernel's average still rests on at least 64 launches where the run has them.
 PROF_EVERY = 16
+
+
+def prof_every(steps):
+    return max(1, min(PROF_EVERY, steps // 64))
 
 
 def timed_run(lat, warmup, steps):
     lat.lf_evolve(warmup)
     lat.synchronize()
-    lat.profile_enable(True, every=PROF_EVERY)
+    lat.profile_enable(True, every=prof_every(steps))
     t0 = time.perf_counter()
     lat.lf_evolve(steps)
     lat.synchronize()
@@ -609,7 +614,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "traffic_source": TRAFFIC_FILE if traffic is not None else None,
                 "kernel": r["kernel"] + " (fused drift+kick+reductions)",
-                "avg_launch_us": avg_s * 1e6, "timed_launches": r["launches"], "timed_every": PROF_EVERY,
+                "avg_launch_us": avg_s * 1e6, "timed_launches": r["launches"], "timed_every": prof_every(args.steps),
                 "bytes_per_launch": r["bytes_per_launch"], "autotune_us": r["autotune_us"]}
 
     if args.calibrate_copy:
