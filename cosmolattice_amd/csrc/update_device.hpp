@@ -14,19 +14,21 @@ namespace clmi {
 __device__ inline double sf_kernel_of(double a, double pi2S, double grad2S, double pi2CS, double grad2CS,
                                       double pi2U1, double mag2U1, double pot, const ModelConsts &mc)
 {
+  // a^-2, a^-4, a^-6 from one division (a serial chain on the updating thread)
+  const double ia = 1.0 / a, ia2 = ia * ia, ia4 = ia2 * ia2, ia6 = ia4 * ia2;
   double Eks = 0, Egs = 0, Ekcs = 0, Egcs = 0, Eel = 0, Emag = 0;
   if (mc.ns > 0) {
-    Eks = 0.5 * pi2S * rpow(a, -6.0);
-    Egs = 0.5 * grad2S * rpow(a, -2.0);
+    Eks = 0.5 * pi2S * ia6;
+    Egs = 0.5 * grad2S * ia2;
   }
   if (mc.ncs > 0) {
-    Ekcs = pi2CS * rpow(a, -6.0);
-    Egcs = grad2CS * rpow(a, -2.0);
+    Ekcs = pi2CS * ia6;
+    Egcs = grad2CS * ia2;
   }
   if (mc.nu1 > 0) {
     const double w2 = (mc.omegaStar / mc.fStar) * (mc.omegaStar / mc.fStar);
-    Eel = 0.5 * w2 * rpow(a, -4.0) * pi2U1;
-    Emag = 0.5 * w2 * rpow(a, -4.0) * mag2U1;
+    Eel = 0.5 * w2 * ia4 * pi2U1;
+    Emag = 0.5 * w2 * ia4 * mag2U1;
   }
   const double fm = mc.fStar / mc.MPl;
   return rpow(a, 2 * mc.alpha + 1) / 3.0 * (fm * fm) *
@@ -214,13 +216,16 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
     }
     if (o == 0 && ops.snap) *ops.snap = b;
   }
-  st->aK = rpow(b.aI, 1 + mc.alpha);
-  st->aV = rpow(b.aI, 3 + mc.alpha);
+  // a^(1+alpha) and a^(3+alpha) = a^(1+alpha) a^2, a^(alpha-1); the drift factors likewise share one power
+  const double aK = rpow(b.aI, 1 + mc.alpha);
+  st->aK = aK;
+  st->aV = aK * (b.aI * b.aI);
   st->aG = rpow(b.aI, mc.alpha - 1);
-  st->aJ = rpow(b.aI, 1 + mc.alpha);
+  st->aJ = aK;
   const double ad = a_drift > 0.0 ? a_drift : b.aSI;
-  st->fdrift_s = rpow(ad, mc.alpha - 3);
-  st->fdrift_u = rpow(ad, mc.alpha - 1);
+  const double fu = rpow(ad, mc.alpha - 1);
+  st->fdrift_u = fu;
+  st->fdrift_s = fu / (ad * ad);
 }
 
 }  // namespace clmi
