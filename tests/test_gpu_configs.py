@@ -13,6 +13,7 @@
 * A model with a real singlet next to the charged scalar + U(1) (both sectors in one pass and one
   background update): oracle parity at small N, and at 512^3 FP32-bulk, where the two sectors' partial
   sums need more rows than either alone.
+* The largest accepted lattice, 1024^3, in both precisions: the default pass against the generic pass.
 
 The 256^3 FP32-bulk vs FP64 comparison of the two-field lphi4 run over its full 4000 steps lives in
 test_gpu_accuracy.py.
@@ -415,3 +416,38 @@ def test_singlet_plus_u1_512_f32_partial_rows(gpu):
     for key in ("pi2AvI", "CSpi2AvI", "U1pi2AvI", "grad2AvI", "CSgrad2AvI", "potAvI"):
         assert b0[key] == pytest.approx(b1[key], rel=1e-9), key
     assert relerr(f0, f1) < 1e-6 and relerr(p0, p1) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [CLMI_F32_BULK, CLMI_F64])
+def test_max_size_1024_default_pass_matches_generic(gpu, prec):
+    """The largest lattice the library accepts, N = 1024 (2^30 sites, the limit of 32-bit site indexing;
+    51.5 GB of lattice arrays in FP32-bulk, 68.7 GB in FP64).  Two-field lphi4 vacuum initial conditions
+    drawn on the device (the same seed twice), then 6 leapfrog steps with expansion through the default
+    pass (the tiled pass; 65536 partial-sum rows per update) and through the generic pass.  The fields and
+    momenta agree bitwise, compared through their mean, mean square and mean squared forward gradient,
+    computed on the device by one kernel; the background agrees to summation-order rounding; the energy
+    constraint stays at the leapfrog's own level.  Nothing of N^3 size crosses to the host."""
+    say(f"case max size 1024 prec={prec}")
+    spec = lphi4(100.0, n_fields=2, phi0_gev=5.6964e18, lam=1.0, pi0_program=-0.5)
+    # the 256^3 workload's spacing and fluctuation cut-off (dx = 0.49, kCutOff = N kIR / 4 = 3.2) in a box
+    # 4 times longer
+    N, kIR, dt = 1024, 0.0125, 0.005
+    res = []
+    for flags in (0, CLMI_FLAG_NO_TILED):
+        lat, _ = vacuum_lattice(spec, N, kIR, dt, prec, seed=77, flags=flags)
+        plan = lat.plan()
+        say("plan", plan.replace("\n", "; "))
+        assert plan.split()[0] == ("k_scalar_tiled" if flags == 0 else "k_scalar_pass"), plan
+        lat.lf_evolve(6)
+        lat.lf_sync()
+        stats = [lat.field_stats(k, f, 0) for k in (FLD_S, PI_S) for f in range(2)]
+        res.append((stats, lat.background(), lat.hubble_constraint()))
+        say("stats", stats[0], "aI", res[-1][1]["aI"], "constraint", res[-1][2])
+        lat.close()
+    (s0, b0, h0), (s1, b1, h1) = res
+    assert s0 == s1
+    for key in ("aI", "aSI", "aDotI", "pi2AvI", "grad2AvI", "potAvI"):
+        assert b0[key] == pytest.approx(b1[key], rel=1e-11), key
+    assert b0["aI"] > 1.0
+    # (the 256^3 run's constraint stays below 1.6e-4 over its 4000 steps; here 7.1e-5 after 6)
+    assert abs(h0[0]) < 2e-4 and h0[0] == pytest.approx(h1[0], rel=1e-6, abs=1e-12)
