@@ -63,7 +63,7 @@ def adjoint_grads(spec, coeffs, phi0, pi0, nsteps, precision=CLMI_F64, expansion
     phiT, piT, bgT = leapfrog_run(c, p, q, spec, N, DX, DT, nsteps, precision, expansion, checkpoint_every)
     L = chi_energy_fraction_and_band(phiT, piT, bgT, c, spec, DX, KIR, BAND)
     L.backward()
-    return float(L), c.grad.numpy(), p.grad.numpy(), q.grad.numpy()
+    return float(L.detach()), c.grad.numpy(), p.grad.numpy(), q.grad.numpy()
 
 
 @pytest.mark.parametrize("expansion", [True, False])
