@@ -552,8 +552,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the lattice on a single MI355X "
-              "(multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library runs each lattice on a single MI355X "
+              "(sharding a lattice over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     from cosmolattice_amd import clmi
