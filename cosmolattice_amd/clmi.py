@@ -33,6 +33,7 @@ CLMI_FLAG_NO_TILED = 1
 CLMI_FLAG_SHAPE_ALT = 2
 CLMI_FLAG_SHAPE_TY4 = 4
 CLMI_FLAG_GAUGE_TY8 = 16
+CLMI_FLAG_NO_DOUBLE_KICK = 32
 
 
 def CLMI_FLAG_MEMMODE(m: int) -> int:

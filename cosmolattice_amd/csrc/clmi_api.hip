@@ -252,20 +252,27 @@ GaugePassArgs gauge_args(const clmi_ctx *c, int fi, int fo, int pi, int po, doub
 }
 
 // Launch one lattice pass over every sector; returns the number of partial rows written in *rows.
-int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows)
+// ckick2_s != 0 (drift + kick passes of scalar-singlet models on the full-row pass): kick the scalars a
+// second time, by ckick2_s, from the same drifted fields (see double_kick_ok).
+int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g, int *rows,
+             double ckick2_s = 0.0)
 {
+  const bool kick2 = ckick2_s != 0.0;
+  if (kick2 && !(drift && kick && c->rowk && c->cfg.n_cscalars == 0))
+    return fail(c, CLMI_E_INVALID, "double kick outside a scalar-singlet full-row drift + kick pass");
   const bool timed = c->prof && drift && kick && (c->prof_count++ % c->prof_every == 0);
   if (timed) CHK(prof_event(c));
   int r = 0;
   if (c->cfg.n_scalars > 0) {
     const int fi = c->cur_fS, fo = drift ? 1 - c->cur_fS : c->cur_fS;
     const int pi = c->cur_pS, po = (drift && kick) ? 1 - c->cur_pS : c->cur_pS;
-    const ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    ScalarPassArgs a = scalar_args(c, fi, fo, pi, po, cdt_s, ckick_s, r);
+    a.ckick2 = ckick2_s;
     const int nrows = c->rowk ? c->rplan.rows : c->tiled ? c->tplan.rows : c->blocks_s;
     if (red && (size_t)r + nrows > c->partial_rows)
       return fail(c, CLMI_E_INVALID, "partial-sum buffer too small for the scalar pass");
     if (c->rowk) {
-      HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream));
+      HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream, kick2));
       if (red) r += c->rplan.rows;
     } else if (c->tiled) {
       HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->tplan, c->stream));
@@ -317,10 +324,10 @@ int update(clmi_ctx *c, int rows, std::initializer_list<std::pair<int, double>> 
 // context holds a snapshot target (c->snap) and the update starts with storeFieldsAverages, the
 // background after that operation -- the state a flush at this point would leave -- is written there.
 int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double ckick_s, double ckick_g,
-                bool do_update, std::initializer_list<std::pair<int, double>> ops)
+                bool do_update, std::initializer_list<std::pair<int, double>> ops, double ckick2_s = 0.0)
 {
   int rows = 0;
-  CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows));
+  CHK(run_pass(c, drift, kick, red, cdt_s, ckick_s, ckick_g, &rows, ckick2_s));
   if (do_update) {
     UpdateOps u = make_ops(ops);
     if (c->snap && u.n > 0 && u.op[0] == OP_STORE_FIELDS) {
@@ -330,6 +337,12 @@ int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, doub
     HIPCHK(c, launch_update(c->dstate, c->partials, rows, c->mc, u, c->stream));
   }
   return CLMI_OK;
+}
+
+// Whether velocity-Verlet stages may share a double-kick pass: scalar singlets only, on the full-row pass.
+bool double_kick_ok(const clmi_ctx *c)
+{
+  return c->rowk && c->cfg.n_scalars > 0 && c->cfg.n_cscalars == 0 && !(c->cfg.flags & CLMI_FLAG_NO_DOUBLE_KICK);
 }
 
 // Apply the deferred drift of the last leapfrog step (drift-only pass + storeFieldsAverages).
@@ -734,6 +747,33 @@ int clmi_vv_evolve(clmi_ctx *c, const double *ws, int nws, int nsteps)
   const double dt = c->cfg.dt;
   const int stages = (nws - 1) * 2 + 1;
   auto weight = [&](int s) { return ws[s < nws ? s : stages - s - 1]; };
+  if (double_kick_ok(c)) {
+    // The second half-kick of a stage and the first half-kick of the next read the same drifted fields,
+    // and the scale-factor operations between them change aDot but not a_I, so both kicks have the
+    // same coefficients: one double-kick pass gives exactly the momenta, and in its two sums exactly
+    // the averages, of the two passes they replace (velocityverlet.h:51-114).
+    const int K = nsteps * stages;
+    auto wk = [&](int k) { return weight(k % stages); };
+    if (K > 0) {
+      if (ex) CHK(update(c, 0, {{OP_VV_SF_HALF, wk(0)}}));
+      CHK(pass_update(c, false, true, ex, 0.0, wk(0) * dt / 2, dt / 2 * wk(0), ex,
+                      {{OP_VV_MOMENTA, 0.0}, {OP_VV_DRIFT_SF, wk(0)}}));
+    }
+    for (int k = 0; k < K; ++k) {
+      const double w = wk(k);
+      if (k + 1 < K) {
+        const double w2 = wk(k + 1);
+        CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
+                        {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}, {OP_VV_SF_HALF, w2},
+                         {OP_VV_MOMENTA_KICK2, 0.0}, {OP_VV_DRIFT_SF, w2}},
+                        w2 * dt / 2));
+      } else {
+        CHK(pass_update(c, true, true, ex, dt * w, w * dt / 2, dt / 2 * w, ex,
+                        {{OP_STORE_FIELDS, 0.0}, {OP_VV_MOMENTA, 0.0}, {OP_VV_SF_ONE, w}}));
+      }
+    }
+    return CLMI_OK;
+  }
   for (int n = 0; n < nsteps; ++n)
     for (int s = 0; s < stages; ++s) {
       const double w = weight(s);

@@ -30,7 +30,7 @@ enum RedSlot : int {
   RED_GRAD2CS = 4,
   RED_PI2U1 = 5,
   RED_B2U1 = 6,
-  RED_SPARE = 7,
+  RED_PI2S_KICK2 = 7,  // scalar momenta after the second kick of a double-kick pass
 };
 
 // Device-resident homogeneous state plus the per-pass coefficients derived from it.
@@ -86,6 +86,7 @@ enum UpdateOp : int {
   OP_PV_MID = 12,        // storeFieldsAveragesHalf + storeMomentaAverages + kickScaleFactor +
                          // driftScaleFactorSecondHalf (positionverlet.h:73-94)
   OP_PV_DRIFT_AI = 13,   // drift coefficients from aI (positionverlet.h:161-192)
+  OP_VV_MOMENTA_KICK2 = 14,  // storeMomentaAverages of the second kick of a double-kick pass
 };
 
 struct UpdateOps {
@@ -115,6 +116,7 @@ struct ScalarPassArgs {
   double inv_dx;
   double cdt;    // drift: phi += fdrift_s * (cdt * pi)
   double ckick;  // kick: pi += ckick * (aK * lap - aV * dV)
+  double ckick2; // double-kick pass: then pi += ckick2 * (aK * lap - aV * dV) with the same fields
   const DevState *st;
   double *partials;  // this pass's rows, kNumRed doubles each
   ScalarPoly poly;
@@ -321,8 +323,9 @@ bool scalar_rows_plan(int N, int nf, bool f32, int XC, int mm, int shape, RowsPl
 // exponent table is lphi4's, on the shapes instantiated with it: FP32-bulk main shapes, FP64 512-thread
 // shapes), else 0 (runtime table).
 int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p);
+// kick2 (with drift and kick): kick a second time, by a.ckick2, from the same drifted fields.
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                              const RowsPlan &p, hipStream_t s);
+                              const RowsPlan &p, hipStream_t s, bool kick2 = false);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                                const TiledPlan &p, hipStream_t s);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);

@@ -5,13 +5,13 @@
 namespace clmi {
 
 hipError_t launch_scalar_rows_f32_nf2(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
-                                      hipStream_t s);
+                                      hipStream_t s, bool kick2);
 hipError_t launch_scalar_rows_f32_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
-                                      hipStream_t s);
+                                      hipStream_t s, bool kick2);
 hipError_t launch_scalar_rows_f64_nf2(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
-                                      hipStream_t s);
+                                      hipStream_t s, bool kick2);
 hipError_t launch_scalar_rows_f64_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p,
-                                      hipStream_t s);
+                                      hipStream_t s, bool kick2);
 
 // Row-kernel shapes by (N, precision): VZ sites per lane, WPR wavefronts per row, TY rows per
 // workgroup.  The shape fixes the summation order; the x-chunk XC and store mode are tuned at run
@@ -55,12 +55,14 @@ int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p)
 }
 
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                              const RowsPlan &p, hipStream_t s)
+                              const RowsPlan &p, hipStream_t s, bool kick2)
 {
   if (nf == 2)
-    return f32 ? launch_scalar_rows_f32_nf2(drift, kick, red, a, p, s) : launch_scalar_rows_f64_nf2(drift, kick, red, a, p, s);
+    return f32 ? launch_scalar_rows_f32_nf2(drift, kick, red, a, p, s, kick2)
+               : launch_scalar_rows_f64_nf2(drift, kick, red, a, p, s, kick2);
   if (nf == 1)
-    return f32 ? launch_scalar_rows_f32_nf1(drift, kick, red, a, p, s) : launch_scalar_rows_f64_nf1(drift, kick, red, a, p, s);
+    return f32 ? launch_scalar_rows_f32_nf1(drift, kick, red, a, p, s, kick2)
+               : launch_scalar_rows_f64_nf1(drift, kick, red, a, p, s, kick2);
   return hipErrorInvalidValue;
 }
 

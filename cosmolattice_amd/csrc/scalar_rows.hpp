@@ -192,8 +192,12 @@ template <int NF, bool F32, int VZ, int WPR, int TY>
 using RowsTile = typename Tr<F32>::V[2][NF][TY + 2][64 * VZ * WPR];
 
 // One workgroup's work: workgroup b of the nwg that cover this lattice.  PK: the potential's
-// compile-time exponent table (0: runtime table).
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK = 0>
+// compile-time exponent table (0: runtime table).  KICK2 (with DRIFT and KICK): after the kick by
+// a.ckick, kick again by a.ckick2 from the same drifted fields -- the kick a following kick-only pass
+// would apply, with the same bits -- and sum the momenta after each kick (slots RED_PI2S and
+// RED_PI2S_KICK2); only the twice-kicked momenta are stored.
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK = 0,
+          bool KICK2 = false>
 __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC, const int G, const int b,
                                           const int nwg, RowsTile<NF, F32, VZ, WPR, TY> &ldsbuf)
 {
@@ -220,6 +224,9 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   const float cf = DRIFT ? (float)(a.st->fdrift_s * a.cdt) : 0.0f;
   const float kK = KICK ? (float)(a.ckick * a.st->aK * a.inv_dx2) : 0.0f;
   const float kV = KICK ? (float)(a.ckick * a.st->aV) : 0.0f;
+  const float kK2 = KICK2 ? (float)(a.ckick2 * a.st->aK * a.inv_dx2) : 0.0f;
+  const float kV2 = KICK2 ? (float)(a.ckick2 * a.st->aV) : 0.0f;
+  static_assert(!KICK2 || (DRIFT && KICK), "a double kick follows a drift");
 
   auto wrapx = [&](int x) -> uint32_t { return (uint32_t)(x < 0 ? x + N : (x >= N ? x - N : x)); };
   auto rowbase = [&](int x, int yy) -> uint32_t {
@@ -243,7 +250,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   V prev[NF][VZ], cur[NF][VZ], nxt[NF][VZ];
   typename T::P pic[NF][VZ], pin[NF][VZ];
   Raw pf[NF], hq[NF];
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
 
   {
   auto &lds = ldsbuf[0];
@@ -296,7 +303,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
       zr[f] = lds[f][row + 1][zb + VZ == N ? 0 : zb + VZ];
     }
 
-    double p2 = 0.0, g2 = 0.0, vsum = 0.0;
+    double p2 = 0.0, g2 = 0.0, vsum = 0.0, p2b = 0.0;
     typename T::P pout[NF][VZ];
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
@@ -304,7 +311,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
       float phif[NF];
       double lapd[NF];
       float lapf[NF];
-      float g2f = 0.0f, p2f = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
+      float g2f = 0.0f, p2f = 0.0f, p2fb = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const V c = cur[f][k];
@@ -345,14 +352,30 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
       for (int f = 0; f < NF; ++f) {
         if constexpr (!F32) {
           double p = pic[f][k];
-          if (KICK) p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
+          if constexpr (KICK2) {
+            const double F = aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f);
+            p += a.ckick * F;
+            p2 += p * p;
+            p += a.ckick2 * F;
+            p2b += p * p;
+          } else {
+            if (KICK) p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
+            p2 += p * p;
+          }
           pout[f][k] = p;
-          p2 += p * p;
         } else {
           float p = pic[f][k];
-          if (KICK) p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
+          if constexpr (KICK2) {
+            const float d = dpot<NF, PK, float>(a.poly, phif, f);
+            p += kK * lapf[f] - kV * d;
+            p2f += p * p;
+            p += kK2 * lapf[f] - kV2 * d;
+            p2fb += p * p;
+          } else {
+            if (KICK) p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
+            p2f += p * p;
+          }
           pout[f][k] = p;
-          p2f += p * p;
         }
       }
       if (RED) {
@@ -360,6 +383,7 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
           vsum += pot<NF, PK, double>(a.poly, phid);
         } else {
           p2 += (double)p2f;
+          if constexpr (KICK2) p2b += (double)p2fb;
           g2 += (double)g2f * a.inv_dx2;
           vsum += (double)pot<NF, PK, float>(a.poly, phif);
         }
@@ -388,12 +412,14 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
       acc0 += p2;
       acc1 += g2;
       acc2 += vsum;
+      if constexpr (KICK2) acc3 += p2b;
     }
     if (RED && (i + 1) % G == 0) {
       double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
+      v[RED_PI2S_KICK2] = acc3;
       const size_t prow = (size_t)(x / G) * nTy + ty;
       block_reduce_rows<NW, kNumRed>(v, a.partials + prow * kNumRed);
-      acc0 = acc1 = acc2 = 0.0;
+      acc0 = acc1 = acc2 = acc3 = 0.0;
     }
     if (more) {
       // the other buffer was last read in the previous iteration, which every wavefront finished
@@ -419,19 +445,31 @@ __device__ __forceinline__ void rows_body(const ScalarPassArgs &a, const int XC,
   }
 }
 
-template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK>
+template <int NF, bool F32, int VZ, int WPR, int TY, bool DRIFT, bool KICK, bool RED, int MM, int PK,
+          bool KICK2 = false>
 __global__ __launch_bounds__(64 * WPR * TY) void k_scalar_rows(const ScalarPassArgs a, const int XC, const int G)
 {
   __shared__ RowsTile<NF, F32, VZ, WPR, TY> ldsbuf;
-  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, MM, PK>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
+  rows_body<NF, F32, VZ, WPR, TY, DRIFT, KICK, RED, MM, PK, KICK2>(a, XC, G, blockIdx.x, gridDim.x, ldsbuf);
 }
 
+// kick2: the double-kick form of the drift + kick pass.
 template <int NF, bool F32, int VZ, int WPR, int TY, int MM, int PK = 0>
 hipError_t dispatch_rows(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
-                         hipStream_t s)
+                         hipStream_t s, bool kick2 = false)
 {
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
   const dim3 blk(64 * WPR * TY);
+  if (kick2) {
+    if (!drift || !kick) return hipErrorInvalidValue;
+    if (red)
+      hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, true, true, true, MM, PK, true>), dim3(blocks), blk, 0, s,
+                         a, XC, G);
+    else
+      hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, true, true, false, MM, PK, true>), dim3(blocks), blk, 0,
+                         s, a, XC, G);
+    return hipGetLastError();
+  }
 #define CLMI_SR(D, K, R)                                                                                       \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                             \
     hipLaunchKernelGGL((k_scalar_rows<NF, F32, VZ, WPR, TY, D, K, R, MM, PK>), dim3(blocks), blk, 0, s, a, XC, G); \

@@ -3,17 +3,18 @@
 
 namespace clmi {
 
-hipError_t launch_scalar_rows_f64_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p, hipStream_t s)
+hipError_t launch_scalar_rows_f64_nf1(bool drift, bool kick, bool red, const ScalarPassArgs &a, const RowsPlan &p, hipStream_t s,
+                                      bool kick2)
 {
   // The lphi4 potential has a compile-time form for the 512-thread shapes (64^3 to 192^3); the
   // 1024-thread shapes are at the 128-VGPR cap, where it would add spills.
   const int pk = scalar_rows_pk(1, false, a.poly, p);
 #define CLMI_MM(NF_, VZ_, WPR_, TY_, PK_)                                                                \
   switch (p.MM) {                                                                                      \
-    case 1: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 1, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
-    case 7: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 7, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
-    case 3: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 3, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s);   \
-    case 15: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 15, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s); \
+    case 1: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 1, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);   \
+    case 7: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 7, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);   \
+    case 3: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 3, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);   \
+    case 15: return dispatch_rows<NF_, false, VZ_, WPR_, TY_, 15, PK_>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2); \
     default: return hipErrorInvalidValue;                                                              \
   }
 #define CLMI_SHAPE(NF_, VZ_, WPR_, TY_)                                                                  \

@@ -150,6 +150,9 @@ __device__ inline void apply_update(DevState *st, const double *red, const Model
         if (mc.ncs > 0) b.CSpi2AvI = red[RED_PI2CS];
         if (mc.nu1 > 0) b.U1pi2AvI = red[RED_PI2U1];
         break;
+      case OP_VV_MOMENTA_KICK2:  // velocityverlet.h:249-256, scalar singlets only
+        b.pi2AvI = red[RED_PI2S_KICK2];
+        break;
       case OP_VV_DRIFT_SF:  // velocityverlet.h:239-242
         b.aIM = b.aI;
         b.aI += dt * b.aDotSI * w;

@@ -52,6 +52,11 @@ extern "C" {
 /* clmi_config.flags: 8-row tiles (512-thread workgroups) in the tiled charged-scalar + U(1) pass
  * instead of 4-row tiles.  Fields bit-identical; volume sums differ in summation order only. */
 #define CLMI_FLAG_GAUGE_TY8 16
+/* clmi_config.flags: run velocity Verlet as two passes per stage.  By default, for scalar-singlet
+ * models on the full-row lattice pass, the second half-kick of each stage and the first half-kick of
+ * the next run in one pass (they read the same fields with the same coefficients); the results are
+ * bit-identical either way.  The flag exists for testing. */
+#define CLMI_FLAG_NO_DOUBLE_KICK 32
 /* clmi_config.flags: memory mode m of the full-row lattice pass (0 = the library's default, 15; 1 =
  * non-temporal stores, 3 = also non-temporal loads of the momenta, 7 = also of the workgroup's own
  * field rows, 15 = as 7 but ordinary loads for the tile's first and last rows).  Results do not

@@ -269,6 +269,41 @@ def test_verlet_passes_bit_identical_to_generic(gpu, N, prec, mm):
             assert np.array_equal(a, b), scheme
 
 
+@pytest.mark.parametrize("N,prec,n_fields,scheme", [(64, CLMI_F64, 1, "VV2"), (64, CLMI_F32_BULK, 2, "VV4"),
+                                                    (128, CLMI_F32_BULK, 2, "VV2"), (128, CLMI_F64, 2, "VV6"),
+                                                    (256, CLMI_F32_BULK, 2, "VV2")])
+def test_vv_double_kick_bit_identical_to_two_passes(gpu, N, prec, n_fields, scheme):
+    """Velocity Verlet with the second half-kick of each stage and the first half-kick of the next in one
+    double-kick pass (the default on the full-row pass) gives the same bits as the two separate passes
+    (CLMI_FLAG_NO_DOUBLE_KICK): fields, momenta and the whole background, with expansion, after three
+    calls of uneven length (the first and last stage of a call stay single kicks) and a download
+    between them."""
+    from cosmolattice_amd.clmi import CLMI_FLAG_NO_DOUBLE_KICK, VV_WEIGHTS
+
+    say(f"case vv double kick N={N} prec={prec} nf={n_fields} {scheme}")
+    spec = bench_spec(n_fields=n_fields)
+    rng = np.random.default_rng(5 * N + prec + n_fields)
+    ics = scalar_ics(spec, N, rng)
+    res = []
+    for flags in (0, CLMI_FLAG_NO_DOUBLE_KICK):
+        lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=True, flags=flags)
+        assert_plan(lat, "k_scalar_rows")
+        for (k, i), a in ics.items():
+            lat.upload(k, i, 0, a)
+        lat.set_background(aDotI=0.6)
+        snaps = []
+        for n in (5, 1, 4):
+            lat.vv_evolve(VV_WEIGHTS[scheme], n)
+            snaps.append(([lat.download(k, i, 0) for (k, i) in sorted(ics)], lat.background()))
+        res.append(snaps)
+        lat.close()
+    for j, ((fa, ba), (fb, bb)) in enumerate(zip(res[0], res[1])):
+        for a, b in zip(fa, fb):
+            assert np.array_equal(a, b), j
+        assert ba == bb, j
+    assert res[0][-1][1]["aI"] > 1.0
+
+
 @pytest.mark.parametrize("prec", [CLMI_F64, CLMI_F32_BULK])
 def test_autotune_keeps_state_and_results(gpu, prec):
     """Autotuning runs passes on the live data but leaves fields and background untouched, and the
