@@ -258,8 +258,8 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
              double ckick2_s = 0.0)
 {
   const bool kick2 = ckick2_s != 0.0;
-  if (kick2 && !(drift && kick && c->rowk && c->cfg.n_cscalars == 0))
-    return fail(c, CLMI_E_INVALID, "double kick outside a scalar-singlet full-row drift + kick pass");
+  if (kick2 && !(drift && kick && (c->rowk || c->tiled) && c->cfg.n_cscalars == 0))
+    return fail(c, CLMI_E_INVALID, "double kick outside a scalar-singlet full-row or tiled drift + kick pass");
   const bool timed = c->prof && drift && kick && (c->prof_count++ % c->prof_every == 0);
   if (timed) CHK(prof_event(c));
   int r = 0;
@@ -275,7 +275,7 @@ int run_pass(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, double 
       HIPCHK(c, launch_scalar_rows(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->rplan, c->stream, kick2));
       if (red) r += c->rplan.rows;
     } else if (c->tiled) {
-      HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->tplan, c->stream));
+      HIPCHK(c, launch_scalar_tiled(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->tplan, c->stream, kick2));
       if (red) r += c->tplan.rows;
     } else {
       HIPCHK(c, launch_scalar_pass(c->cfg.n_scalars, c->f32, drift, kick, red, a, c->blocks_s, c->stream));
@@ -339,10 +339,11 @@ int pass_update(clmi_ctx *c, bool drift, bool kick, bool red, double cdt_s, doub
   return CLMI_OK;
 }
 
-// Whether velocity-Verlet stages may share a double-kick pass: scalar singlets only, on the full-row pass.
+// Whether velocity-Verlet stages may share a double-kick pass: scalar singlets only, on the full-row or
+// the tiled pass.
 bool double_kick_ok(const clmi_ctx *c)
 {
-  return c->rowk && c->cfg.n_scalars > 0 && c->cfg.n_cscalars == 0 && !(c->cfg.flags & CLMI_FLAG_NO_DOUBLE_KICK);
+  return (c->rowk || c->tiled) && c->cfg.n_scalars > 0 && c->cfg.n_cscalars == 0 && !(c->cfg.flags & CLMI_FLAG_NO_DOUBLE_KICK);
 }
 
 // Apply the deferred drift of the last leapfrog step (drift-only pass + storeFieldsAverages).

@@ -327,7 +327,7 @@ int scalar_rows_pk(int nf, bool f32, const ScalarPoly &P, const RowsPlan &p);
 hipError_t launch_scalar_rows(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
                               const RowsPlan &p, hipStream_t s, bool kick2 = false);
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                               const TiledPlan &p, hipStream_t s);
+                               const TiledPlan &p, hipStream_t s, bool kick2 = false);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 // Copies of up to kCopySetMax device arrays in one launch (sizes in 16-byte units).
 constexpr int kCopySetMax = 12;

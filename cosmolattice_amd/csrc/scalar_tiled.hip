@@ -93,9 +93,13 @@ __device__ __forceinline__ void store_out(S *p, uint32_t i, S v)
 // Partial sums are written per (group of G planes, y-tile, z-tile), independent of the x-chunk
 // length XC (a multiple of G), so every XC gives the same summation order.
 // PK: the potential's compile-time exponent table (0: runtime table).
-template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY, bool NT, int PK>
+// KICK2 (with DRIFT and KICK): kick a second time, by a.ckick2, from the same drifted fields, summing the
+// momenta after each kick (slots RED_PI2S and RED_PI2S_KICK2) and storing only the twice-kicked ones; each
+// product and sum is formed as a following kick-only pass would form it.
+template <int NF, bool F32, bool DRIFT, bool KICK, bool RED, int RY, bool NT, int PK, bool KICK2 = false>
 __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a, const int XC, const int G)
 {
+  static_assert(!KICK2 || (DRIFT && KICK), "a double kick follows a drift");
   using T = Tr<F32>;
   using V = typename T::V;
   using Raw = typename T::Raw;
@@ -119,6 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
   const float cf = DRIFT ? (float)(a.st->fdrift_s * a.cdt) : 0.0f;
   const float kK = KICK ? (float)(a.ckick * a.st->aK * a.inv_dx2) : 0.0f;
   const float kV = KICK ? (float)(a.ckick * a.st->aV) : 0.0f;
+  const float kK2 = KICK2 ? (float)(a.ckick2 * a.st->aK * a.inv_dx2) : 0.0f;
+  const float kV2 = KICK2 ? (float)(a.ckick2 * a.st->aV) : 0.0f;
   const bool need_p_nb = DRIFT;  // momenta of neighbouring planes are read only to drift them
 
   auto site = [&](int x, int y, int zz) -> uint32_t {
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
   V prev[RY][NF], cur[RY][NF], nxt[RY][NF];
   typename T::P pic[RY][NF], pin[RY][NF];
   Raw pf[RY][NF];
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
 
   // prologue: planes x0-1, x0, x0+1 and the LDS tile of plane x0
 #pragma unroll
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
       const int ly = wave * RY + r + 1;
       const uint32_t s = site(x, y0 + ly - 1, z);
       double g2 = 0.0;
-      float g2f = 0.0f, p2f = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
+      float g2f = 0.0f, p2f = 0.0f, p2fb = 0.0f;  // FP32-bulk: per-site terms in FP32, accumulated in FP64
       double phid[NF];
       float phif[NF];
       double lapd[NF];
@@ -239,26 +245,45 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
           if (RED) g2f += (dxp * dxp + dyp * dyp) + dzp * dzp;
         }
       }
-      double p2 = 0.0;
+      double p2 = 0.0, p2b = 0.0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         if constexpr (!F32) {
           double p = pic[r][f];
-          if (KICK) {
-            p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
+          if constexpr (KICK2) {
+            const double F = aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f);
+            p += a.ckick * F;
+            p2 += p * p;
+            p += a.ckick2 * F;
+            p2b += p * p;
             store_out<NT>((double *)a.pi_out[f], s, p);
+          } else {
+            if (KICK) {
+              p += a.ckick * (aK * lapd[f] - aV * dpot<NF, PK, double>(a.poly, phid, f));
+              store_out<NT>((double *)a.pi_out[f], s, p);
+            }
+            p2 += p * p;
           }
-          p2 += p * p;
         } else {
           float p = pic[r][f];
-          if (KICK) {
-            p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
+          if constexpr (KICK2) {
+            const float d = dpot<NF, PK, float>(a.poly, phif, f);
+            p += kK * lapf[f] - kV * d;
+            p2f += p * p;
+            p += kK2 * lapf[f] - kV2 * d;
+            p2fb += p * p;
             store_out<NT>((float *)a.pi_out[f], s, p);
+          } else {
+            if (KICK) {
+              p += kK * lapf[f] - kV * dpot<NF, PK, float>(a.poly, phif, f);
+              store_out<NT>((float *)a.pi_out[f], s, p);
+            }
+            p2f += p * p;
           }
-          p2f += p * p;
         }
       }
       if (RED) {
+        if constexpr (KICK2) acc3 += F32 ? (double)p2fb : p2b;
         if constexpr (!F32) {
           acc0 += p2;
           acc1 += g2;
@@ -275,9 +300,10 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
       // one partial row per (plane group, y-tile, z-tile); the loop barrier below separates
       // consecutive uses of the reduction's shared scratch
       double v[kNumRed] = {acc0, acc1, acc2, 0, 0, 0, 0, 0};
+      v[RED_PI2S_KICK2] = acc3;
       const size_t row = ((size_t)(x / G) * nTy + ty) * nTz + tz;
       block_reduce_store<kNumRed>(v, a.partials + row * kNumRed);
-      acc0 = acc1 = acc2 = 0.0;
+      acc0 = acc1 = acc2 = acc3 = 0.0;
     }
 
     if (more) {
@@ -308,10 +334,20 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tiled(const ScalarPassArgs a,
 
 template <int NF, bool F32, bool NT, int PK>
 hipError_t dispatch_tiled(bool drift, bool kick, bool red, const ScalarPassArgs &a, int blocks, int XC, int G,
-                          hipStream_t s)
+                          hipStream_t s, bool kick2)
 {
   constexpr int RY = 4;
   const int code = (drift ? 4 : 0) | (kick ? 2 : 0) | (red ? 1 : 0);
+  if (kick2) {
+    if (!drift || !kick) return hipErrorInvalidValue;
+    if (red)
+      hipLaunchKernelGGL((k_scalar_tiled<NF, F32, true, true, true, RY, NT, PK, true>), dim3(blocks), dim3(kBlock), 0, s,
+                         a, XC, G);
+    else
+      hipLaunchKernelGGL((k_scalar_tiled<NF, F32, true, true, false, RY, NT, PK, true>), dim3(blocks), dim3(kBlock), 0,
+                         s, a, XC, G);
+    return hipGetLastError();
+  }
 #define CLMI_ST(D, K, R)                                                                                        \
   case ((D ? 4 : 0) | (K ? 2 : 0) | (R ? 1 : 0)):                                                              \
     hipLaunchKernelGGL((k_scalar_tiled<NF, F32, D, K, R, RY, NT, PK>), dim3(blocks), dim3(kBlock), 0, s, a, XC, G); \
@@ -356,28 +392,28 @@ bool scalar_tiled_plan(int N, int nf, int XC, bool nt, TiledPlan *p)
 }
 
 hipError_t launch_scalar_tiled(int nf, bool f32, bool drift, bool kick, bool red, const ScalarPassArgs &a,
-                               const TiledPlan &p, hipStream_t s)
+                               const TiledPlan &p, hipStream_t s, bool kick2)
 {
   // the compile-time lphi4 potential when the run's exponent table is lphi4's
   const bool pk = poly_kind(a.poly, nf) == 1;
   const int code = (nf == 2 ? 4 : 0) | (f32 ? 2 : 0) | (p.NT ? 1 : 0) | (pk ? 8 : 0);
   switch (code) {
-    case 0: return dispatch_tiled<1, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 1: return dispatch_tiled<1, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 2: return dispatch_tiled<1, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 3: return dispatch_tiled<1, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 4: return dispatch_tiled<2, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 5: return dispatch_tiled<2, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 6: return dispatch_tiled<2, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 7: return dispatch_tiled<2, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 10: return dispatch_tiled<1, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 11: return dispatch_tiled<1, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 14: return dispatch_tiled<2, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 15: return dispatch_tiled<2, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 12: return dispatch_tiled<2, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 13: return dispatch_tiled<2, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 8: return dispatch_tiled<1, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
-    case 9: return dispatch_tiled<1, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s);
+    case 0: return dispatch_tiled<1, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 1: return dispatch_tiled<1, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 2: return dispatch_tiled<1, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 3: return dispatch_tiled<1, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 4: return dispatch_tiled<2, false, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 5: return dispatch_tiled<2, false, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 6: return dispatch_tiled<2, true, false, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 7: return dispatch_tiled<2, true, true, 0>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 10: return dispatch_tiled<1, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 11: return dispatch_tiled<1, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 14: return dispatch_tiled<2, true, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 15: return dispatch_tiled<2, true, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 12: return dispatch_tiled<2, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 13: return dispatch_tiled<2, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 8: return dispatch_tiled<1, false, false, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
+    case 9: return dispatch_tiled<1, false, true, 1>(drift, kick, red, a, p.blocks, p.XC, p.G, s, kick2);
     default: return hipErrorInvalidValue;
   }
 }

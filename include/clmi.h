@@ -53,7 +53,7 @@ extern "C" {
  * instead of 4-row tiles.  Fields bit-identical; volume sums differ in summation order only. */
 #define CLMI_FLAG_GAUGE_TY8 16
 /* clmi_config.flags: run velocity Verlet as two passes per stage.  By default, for scalar-singlet
- * models on the full-row lattice pass, the second half-kick of each stage and the first half-kick of
+ * models on the full-row or tiled lattice pass, the second half-kick of each stage and the first half-kick of
  * the next run in one pass (they read the same fields with the same coefficients); the results are
  * bit-identical either way.  The flag exists for testing. */
 #define CLMI_FLAG_NO_DOUBLE_KICK 32

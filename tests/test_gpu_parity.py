@@ -271,10 +271,11 @@ def test_verlet_passes_bit_identical_to_generic(gpu, N, prec, mm):
 
 @pytest.mark.parametrize("N,prec,n_fields,scheme", [(64, CLMI_F64, 1, "VV2"), (64, CLMI_F32_BULK, 2, "VV4"),
                                                     (128, CLMI_F32_BULK, 2, "VV2"), (128, CLMI_F64, 2, "VV6"),
-                                                    (256, CLMI_F32_BULK, 2, "VV2")])
+                                                    (256, CLMI_F32_BULK, 2, "VV2"), (256, CLMI_F64, 2, "VV4"),
+                                                    (320, CLMI_F32_BULK, 2, "VV2"), (320, CLMI_F64, 1, "VV2")])
 def test_vv_double_kick_bit_identical_to_two_passes(gpu, N, prec, n_fields, scheme):
     """Velocity Verlet with the second half-kick of each stage and the first half-kick of the next in one
-    double-kick pass (the default on the full-row pass) gives the same bits as the two separate passes
+    double-kick pass (the default on the full-row and the tiled pass) gives the same bits as the two separate passes
     (CLMI_FLAG_NO_DOUBLE_KICK): fields, momenta and the whole background, with expansion, after three
     calls of uneven length (the first and last stage of a call stay single kicks) and a download
     between them."""
@@ -287,7 +288,7 @@ def test_vv_double_kick_bit_identical_to_two_passes(gpu, N, prec, n_fields, sche
     res = []
     for flags in (0, CLMI_FLAG_NO_DOUBLE_KICK):
         lat = Lattice(spec, N, 0.4909, 0.005, prec, expansion=True, flags=flags)
-        assert_plan(lat, "k_scalar_rows")
+        assert_plan(lat, expected_scalar_pass(N, prec))
         for (k, i), a in ics.items():
             lat.upload(k, i, 0, a)
         lat.set_background(aDotI=0.6)
